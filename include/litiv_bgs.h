@@ -1,0 +1,147 @@
+/*
+ * litiv_bgs.h -- C ABI of the MI355X (gfx950) SuBSENSE / LOBSTER background subtractors.
+ *
+ * Drop-in boundary for litiv's IBackgroundSubtractor surface
+ * (modules/video/include/litiv/video/BackgroundSubtractionUtils.hpp:24-97): plain pointers and
+ * sizes, no exceptions across the ABI (status codes + per-context message), one context per
+ * video stream. Each entry point names the reference interface it replaces.
+ *
+ * Threading: calls on one context must be serialized; different contexts are independent
+ * (their random streams are keyed by lvbgs_params.stream_id) and may be used concurrently.
+ * Images are 8-bit, 1 or 3 channels, interleaved (HWC), row stride in bytes.
+ */
+#ifndef LITIV_BGS_H
+#define LITIV_BGS_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LVBGS_OK 0
+#define LVBGS_ERR_ARG (-1)      /* invalid argument (the reference's lvAssert_ failures) */
+#define LVBGS_ERR_STATE (-2)    /* "algo & model must be initialized first" */
+#define LVBGS_ERR_HIP (-3)      /* HIP runtime error */
+#define LVBGS_ERR_ALLOC (-4)    /* device memory allocation failed */
+#define LVBGS_ERR_DEVICE (-5)   /* device-side consistency check failed */
+
+enum { LVBGS_SUBSENSE = 1, LVBGS_LOBSTER = 2 };
+
+typedef struct lvbgs_ctx lvbgs_ctx;
+
+/* Constructor parameters (BackgroundSubtractorSuBSENSE.hpp:52-57, BackgroundSubtractorLOBSTER.hpp:50-55,
+ * BackgroundSubtractorLBSP.hpp:46-49), plus the counter-based RNG key and the device. */
+typedef struct lvbgs_params {
+    size_t desc_dist_threshold;   /* SuBSENSE nDescDistThresholdOffset (3) | LOBSTER nDescDistThreshold (4) */
+    size_t color_dist_threshold;  /* SuBSENSE nMinColorDistThreshold (30) | LOBSTER nColorDistThreshold (30) */
+    size_t bg_samples;            /* nBGSamples: 50 | 35 (this build: 1..255) */
+    size_t required_bg_samples;   /* nRequiredBGSamples: 2 */
+    size_t samples_for_mv_avgs;   /* SuBSENSE nSamplesForMovingAvgs (100) */
+    size_t lbsp_threshold_offset; /* nLBSPThresholdOffset (0) */
+    float rel_lbsp_threshold;     /* fRelLBSPThreshold (0.333f) */
+    int median_blur_kernel;       /* nDefaultMedianBlurKernelSize (9), odd, 3..13 */
+    uint64_t rng_seed;            /* replaces the global libc rand() state */
+    uint32_t stream_id;           /* per-stream RNG counter word */
+    int device;                   /* HIP device ordinal */
+} lvbgs_params;
+
+/* Fills the reference defaults of the given algorithm. */
+int lvbgs_default_params(int algo, lvbgs_params* out);
+
+/* Constructor (BackgroundSubtractorSuBSENSE.cpp:62-78 / BackgroundSubtractorLOBSTER.cpp:20-30). */
+int lvbgs_create(int algo, const lvbgs_params* params, lvbgs_ctx** out);
+void lvbgs_destroy(lvbgs_ctx* ctx);
+
+/* IIBackgroundSubtractor::initialize(img, roi) (BackgroundSubtractionUtils.hpp:29-31;
+ * SuBSENSE .cpp:107-186, LOBSTER .cpp:442-457). Host pointers. roi may be NULL (reuse the last ROI
+ * of the same size, else the full frame); otherwise a 0/255 mask of the same size. */
+int lvbgs_initialize(lvbgs_ctx* ctx, const uint8_t* img, int width, int height, int channels, size_t stride,
+                     const uint8_t* roi, size_t roi_stride);
+
+/* IIBackgroundSubtractor::apply(image, fgmask, learningRate) (BackgroundSubtractionUtils.hpp:35;
+ * SuBSENSE .cpp:188-612, LOBSTER .cpp:459-581). Host pointers, synchronous. fgmask receives the
+ * width x height 0/255 mask. SuBSENSE: lr <= 0 uses the per-pixel T(x), lr = +inf never updates;
+ * LOBSTER: lr must be > 0. */
+int lvbgs_apply(lvbgs_ctx* ctx, const uint8_t* img, size_t stride, uint8_t* fgmask, size_t mask_stride, double lr);
+
+/* Same step on device-resident, tightly packed buffers; enqueued on the context's stream and
+ * returns without waiting. d_mask may be NULL. */
+int lvbgs_apply_device(lvbgs_ctx* ctx, const void* d_img, void* d_mask, double lr);
+
+/* Waits for the context's stream; reports device-side errors of the steps since the last call. */
+int lvbgs_synchronize(lvbgs_ctx* ctx);
+
+/* getBackgroundImage (SuBSENSE .cpp:614-630, LOBSTER .cpp:583-600). Host pointer. */
+int lvbgs_get_background_image(lvbgs_ctx* ctx, uint8_t* out, size_t stride);
+
+/* getBackgroundDescriptorsImage (BackgroundSubtractorLBSP.hpp:41; SuBSENSE .cpp:632-649). Host pointer,
+ * 16-bit values, stride in bytes. */
+int lvbgs_get_background_descriptors_image(lvbgs_ctx* ctx, uint16_t* out, size_t stride);
+
+/* setAutomaticModelReset (BackgroundSubtractionUtils.cpp:24-26). */
+int lvbgs_set_automatic_model_reset(lvbgs_ctx* ctx, int enabled);
+
+/* refreshModel(fSamplesRefreshFrac, bForceFGUpdate) (SuBSENSE .cpp:80-105, LOBSTER .cpp:410-440). */
+int lvbgs_refresh_model(lvbgs_ctx* ctx, float frac, int force_fg);
+
+/* getDefaultLearningRate (SuBSENSE .hpp:69 -> 0, LOBSTER .hpp:57 -> 16). */
+double lvbgs_default_learning_rate(const lvbgs_ctx* ctx);
+
+/* getROICopy (BackgroundSubtractionUtils.cpp:50-52). */
+int lvbgs_get_roi(lvbgs_ctx* ctx, uint8_t* out, size_t stride);
+
+/* validateROI (BackgroundSubtractionUtils.cpp:28-36): zeroes the 2-pixel LBSP border in place. */
+int lvbgs_validate_roi(int width, int height, uint8_t* roi, size_t stride);
+
+/* setROI (BackgroundSubtractionUtils.cpp:38-48): validates roi in place; if initialized, re-initializes
+ * from the current background image with the new ROI, otherwise stores it for the next initialize. */
+int lvbgs_set_roi(lvbgs_ctx* ctx, uint8_t* roi, size_t stride);
+
+/* Last error message of the context ("" when none). */
+const char* lvbgs_last_error(const lvbgs_ctx* ctx);
+
+/* LBSP::compute2 with an absolute threshold (LBSP.cpp:102-123, :227-233): device pointers,
+ * tightly packed; border pixels (2 px) are written as 0. stream: hipStream_t or NULL. */
+int lvbgs_lbsp_compute_abs(const void* d_img, int width, int height, int channels, int threshold, void* d_out,
+                           void* stream);
+
+/* ---- introspection (tests, benchmarks) ---- */
+typedef struct lvbgs_stats {
+    uint64_t examined;        /* samples examined by the final ordered match, summed over pixels (k_p) */
+    uint64_t writes;          /* sample writes landing on pixels (w_p) */
+    uint32_t fixpoint_iters;  /* fix-point iterations that had work in the last step */
+    uint32_t max_fixpoint_iter;
+    uint32_t width, height, channels, n_roi;
+} lvbgs_stats;
+int lvbgs_get_stats(lvbgs_ctx* ctx, lvbgs_stats* out);
+
+/* Copies a named internal buffer to host memory (samples_color, samples_desc, last_color, last_desc,
+ * last_fg, roi, lut, T, R, V, MLD, DminLT, DminST, RawLT, RawST, FinLT, FinST, unstable, blinks,
+ * last_raw, last_raw_blink, last_dil_inv, dsLT, dsST). Returns bytes written or < 0. */
+long long lvbgs_get_state(lvbgs_ctx* ctx, const char* name, void* out, size_t cap);
+
+/* Frame-level scalars in the order frame_idx, frames_since_reset, cooldown, auto_reset, lr_scaling,
+ * use3x3, median_k, n_roi, lower_cap, upper_cap, last_nz_ratio, refresh_seq, apply_count. */
+int lvbgs_get_scalars(lvbgs_ctx* ctx, double* out, int cap);
+
+/* hipStream_t of the context (for ordering device-pointer calls with other work). */
+void* lvbgs_stream(lvbgs_ctx* ctx);
+
+/* Kernel timing: stage = "match" | "step" | "" (off). While on, every step records HIP events around
+ * the stage on the context's stream; lvbgs_take_timing returns the summed milliseconds and count since
+ * the last take (it synchronizes). */
+int lvbgs_set_timing(lvbgs_ctx* ctx, const char* stage);
+int lvbgs_take_timing(lvbgs_ctx* ctx, double* total_ms, int* count);
+
+/* Fix-point schedule: full-grid iterations before the single-workgroup tail, and their grid size. */
+int lvbgs_set_fixpoint_schedule(lvbgs_ctx* ctx, int launches, int blocks);
+
+/* RNG self-check: the four draws of group 0 of counter (pixel p, frame, phase, event) for p in [0, n),
+ * written to host_out[4*p .. 4*p+3] (phase 1 = apply, 2 = refresh). */
+int lvbgs_debug_rng(lvbgs_ctx* ctx, uint32_t frame, uint32_t phase, uint32_t event, uint32_t n, uint32_t* host_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

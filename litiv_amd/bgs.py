@@ -1,0 +1,294 @@
+"""Python mirror of litiv's IBackgroundSubtractor for the gfx950 SuBSENSE / LOBSTER library.
+
+Method names follow the reference classes (modules/video/include/litiv/video/
+BackgroundSubtractionUtils.hpp:24-97, BackgroundSubtractorSuBSENSE.hpp:48-135,
+BackgroundSubtractorLOBSTER.hpp:132-155) so code written against the reference reads the same:
+
+    bgs = BackgroundSubtractorSuBSENSE()
+    bgs.initialize(frame0)            # numpy HxWx3 / HxW uint8
+    mask = bgs.apply(frame)           # HxW uint8, 0/255
+    bg = bgs.getBackgroundImage()
+
+Errors raise LvbgsError carrying the library's message (the reference throws lv::Exception).
+Every call goes through liblitiv_bgs.so (include/litiv_bgs.h); there is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liblitiv_bgs.so")
+
+SUBSENSE, LOBSTER = 1, 2
+OK, ERR_ARG, ERR_STATE, ERR_HIP, ERR_ALLOC, ERR_DEVICE = 0, -1, -2, -3, -4, -5
+
+SCALAR_NAMES = ["frame_idx", "frames_since_reset", "cooldown", "auto_reset", "lr_scaling", "use3x3", "median_k",
+                "n_roi", "lower_cap", "upper_cap", "last_nz_ratio", "refresh_seq", "apply_count"]
+
+
+class LvbgsError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("%s (code %d)" % (msg, code))
+        self.code = code
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("desc_dist_threshold", C.c_size_t),
+        ("color_dist_threshold", C.c_size_t),
+        ("bg_samples", C.c_size_t),
+        ("required_bg_samples", C.c_size_t),
+        ("samples_for_mv_avgs", C.c_size_t),
+        ("lbsp_threshold_offset", C.c_size_t),
+        ("rel_lbsp_threshold", C.c_float),
+        ("median_blur_kernel", C.c_int),
+        ("rng_seed", C.c_uint64),
+        ("stream_id", C.c_uint32),
+        ("device", C.c_int),
+    ]
+
+
+class Stats(C.Structure):
+    _fields_ = [("examined", C.c_uint64), ("writes", C.c_uint64), ("fixpoint_iters", C.c_uint32),
+                ("max_fixpoint_iter", C.c_uint32), ("width", C.c_uint32), ("height", C.c_uint32),
+                ("channels", C.c_uint32), ("n_roi", C.c_uint32)]
+
+
+_lib = None
+
+
+def library_path():
+    return _LIB_PATH
+
+
+def lib():
+    """Loads liblitiv_bgs.so; raises if it is missing (build it with __graft_entry__.build())."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            raise RuntimeError("liblitiv_bgs.so is not built (run __graft_entry__.build()); "
+                               "litiv_amd has no CPU fallback")
+        L = C.CDLL(_LIB_PATH)
+        vp, sz, i = C.c_void_p, C.c_size_t, C.c_int
+        L.lvbgs_default_params.argtypes = [i, C.POINTER(Params)]
+        L.lvbgs_create.argtypes = [i, C.POINTER(Params), C.POINTER(vp)]
+        L.lvbgs_destroy.argtypes = [vp]
+        L.lvbgs_initialize.argtypes = [vp, vp, i, i, i, sz, vp, sz]
+        L.lvbgs_apply.argtypes = [vp, vp, sz, vp, sz, C.c_double]
+        L.lvbgs_apply_device.argtypes = [vp, vp, vp, C.c_double]
+        L.lvbgs_synchronize.argtypes = [vp]
+        L.lvbgs_get_background_image.argtypes = [vp, vp, sz]
+        L.lvbgs_get_background_descriptors_image.argtypes = [vp, vp, sz]
+        L.lvbgs_set_automatic_model_reset.argtypes = [vp, i]
+        L.lvbgs_refresh_model.argtypes = [vp, C.c_float, i]
+        L.lvbgs_default_learning_rate.argtypes = [vp]
+        L.lvbgs_default_learning_rate.restype = C.c_double
+        L.lvbgs_get_roi.argtypes = [vp, vp, sz]
+        L.lvbgs_validate_roi.argtypes = [i, i, vp, sz]
+        L.lvbgs_set_roi.argtypes = [vp, vp, sz]
+        L.lvbgs_last_error.argtypes = [vp]
+        L.lvbgs_last_error.restype = C.c_char_p
+        L.lvbgs_lbsp_compute_abs.argtypes = [vp, i, i, i, i, vp, vp]
+        L.lvbgs_get_stats.argtypes = [vp, C.POINTER(Stats)]
+        L.lvbgs_get_state.argtypes = [vp, C.c_char_p, vp, sz]
+        L.lvbgs_get_state.restype = C.c_longlong
+        L.lvbgs_get_scalars.argtypes = [vp, C.POINTER(C.c_double), i]
+        L.lvbgs_stream.argtypes = [vp]
+        L.lvbgs_stream.restype = vp
+        L.lvbgs_set_timing.argtypes = [vp, C.c_char_p]
+        L.lvbgs_take_timing.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int)]
+        L.lvbgs_set_fixpoint_schedule.argtypes = [vp, i, i]
+        L.lvbgs_debug_rng.argtypes = [vp, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, vp]
+        _lib = L
+    return _lib
+
+
+def default_params(algo):
+    p = Params()
+    if lib().lvbgs_default_params(algo, C.byref(p)) != OK:
+        raise ValueError("unknown algorithm %r" % algo)
+    return p
+
+
+def _u8(img):
+    a = np.ascontiguousarray(img)
+    if a.dtype != np.uint8:
+        raise TypeError("images must be uint8 (CV_8UC1 / CV_8UC3)")
+    return a
+
+
+def lbsp_compute_abs(d_img, width, height, channels, threshold, d_out, stream=None):
+    """LBSP::compute2 with an absolute threshold on device buffers (LBSP.cpp:102-123)."""
+    rc = lib().lvbgs_lbsp_compute_abs(C.c_void_p(d_img), width, height, channels, int(threshold), C.c_void_p(d_out),
+                                      C.c_void_p(stream or 0))
+    if rc != OK:
+        raise LvbgsError(rc, "lvbgs_lbsp_compute_abs failed")
+
+
+class BackgroundSubtractor:
+    """IIBackgroundSubtractor (BackgroundSubtractionUtils.hpp:24-97) backed by the HIP library."""
+
+    ALGO = None
+
+    def __init__(self, params=None, **kw):
+        p = params if params is not None else default_params(self.ALGO)
+        for k, v in kw.items():
+            if not hasattr(p, k):
+                raise TypeError("unknown parameter %r" % k)
+            setattr(p, k, v)
+        self.params = p
+        h = C.c_void_p()
+        rc = lib().lvbgs_create(self.ALGO, C.byref(p), C.byref(h))
+        if rc != OK:
+            raise LvbgsError(rc, "lvbgs_create failed: invalid parameters" if rc == ERR_ARG else "lvbgs_create failed")
+        self._h = h
+        self.shape = None
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().lvbgs_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def _check(self, rc):
+        if rc < 0:
+            raise LvbgsError(rc, lib().lvbgs_last_error(self._h).decode())
+        return rc
+
+    # ---- reference interface ----
+    def initialize(self, img, roi=None):
+        img = _u8(img)
+        h, w = img.shape[:2]
+        c = 1 if img.ndim == 2 else img.shape[2]
+        r, rs = None, 0
+        if roi is not None:
+            roi = _u8(roi)
+            r, rs = roi.ctypes.data, roi.strides[0]
+        self._check(lib().lvbgs_initialize(self._h, img.ctypes.data, w, h, c, img.strides[0], r, rs))
+        self.shape = (h, w, c)
+
+    def apply(self, img, fgmask=None, learningRate=None):
+        if learningRate is None:
+            learningRate = self.getDefaultLearningRate()
+        img = _u8(img)
+        if self.shape is None:
+            raise LvbgsError(ERR_STATE, "algo & model must be initialized first")
+        h, w, c = self.shape
+        if img.shape[:2] != (h, w) or (1 if img.ndim == 2 else img.shape[2]) != c:
+            raise LvbgsError(ERR_ARG, "input image type/size mismatch with initialization type/size")
+        if fgmask is None:
+            fgmask = np.empty((h, w), np.uint8)
+        self._check(lib().lvbgs_apply(self._h, img.ctypes.data, img.strides[0], fgmask.ctypes.data, fgmask.strides[0],
+                                      float(learningRate)))
+        return fgmask
+
+    def apply_device(self, d_img, d_mask=None, learningRate=None):
+        """apply() on device-resident buffers (tightly packed); asynchronous on this context's stream."""
+        if learningRate is None:
+            learningRate = self.getDefaultLearningRate()
+        self._check(lib().lvbgs_apply_device(self._h, C.c_void_p(d_img), C.c_void_p(d_mask or 0), float(learningRate)))
+
+    def synchronize(self):
+        self._check(lib().lvbgs_synchronize(self._h))
+
+    def getBackgroundImage(self):
+        h, w, c = self.shape
+        out = np.empty((h, w, c) if c > 1 else (h, w), np.uint8)
+        self._check(lib().lvbgs_get_background_image(self._h, out.ctypes.data, out.strides[0]))
+        return out
+
+    def getBackgroundDescriptorsImage(self):
+        h, w, c = self.shape
+        out = np.empty((h, w, c) if c > 1 else (h, w), np.uint16)
+        self._check(lib().lvbgs_get_background_descriptors_image(self._h, out.ctypes.data, out.strides[0]))
+        return out
+
+    def getDefaultLearningRate(self):
+        return lib().lvbgs_default_learning_rate(self._h)
+
+    def setAutomaticModelReset(self, enabled):
+        self._check(lib().lvbgs_set_automatic_model_reset(self._h, int(bool(enabled))))
+
+    def refreshModel(self, fSamplesRefreshFrac, bForceFGUpdate=False):
+        self._check(lib().lvbgs_refresh_model(self._h, float(fSamplesRefreshFrac), int(bool(bForceFGUpdate))))
+
+    @staticmethod
+    def validateROI(roi):
+        roi = np.ascontiguousarray(roi, dtype=np.uint8).copy()
+        rc = lib().lvbgs_validate_roi(roi.shape[1], roi.shape[0], roi.ctypes.data, roi.strides[0])
+        if rc != OK:
+            raise LvbgsError(rc, "provided ROI must be non-empty and of type 8UC1")
+        return roi
+
+    def setROI(self, roi):
+        roi = np.ascontiguousarray(roi, dtype=np.uint8).copy()
+        self._check(lib().lvbgs_set_roi(self._h, roi.ctypes.data, roi.strides[0]))
+        if self.shape is not None:
+            self.shape = (roi.shape[0], roi.shape[1], self.shape[2])
+        return roi
+
+    def getROICopy(self):
+        h, w, _ = self.shape
+        out = np.empty((h, w), np.uint8)
+        self._check(lib().lvbgs_get_roi(self._h, out.ctypes.data, out.strides[0]))
+        return out
+
+    # ---- introspection (tests / benchmarks) ----
+    def stream(self):
+        return lib().lvbgs_stream(self._h)
+
+    def stats(self):
+        s = Stats()
+        self._check(lib().lvbgs_get_stats(self._h, C.byref(s)))
+        return {k: getattr(s, k) for k, _ in Stats._fields_}
+
+    def scalars(self):
+        buf = (C.c_double * 32)()
+        n = self._check(lib().lvbgs_get_scalars(self._h, buf, 32))
+        return {SCALAR_NAMES[i]: buf[i] for i in range(n)}
+
+    def state(self, name):
+        h, w, c = self.shape
+        n = self.params.bg_samples
+        spec = {
+            "samples_color": (np.uint8, (n, h, w, c)), "samples_desc": (np.uint16, (n, h, w, c)),
+            "last_color": (np.uint8, (h, w, c)), "last_desc": (np.uint16, (h, w, c)), "desc_cur": (np.uint16, (h, w, c)),
+            "lut": (np.uint8, (256,)), "dsLT": (np.float32, (h // 8, w // 8, c)), "dsST": (np.float32, (h // 8, w // 8, c)),
+            "mres": (np.uint32, (h, w)), "rec": (np.uint32, (h, w)),
+        }
+        u8 = {"last_fg", "roi", "unstable", "blinks", "last_raw", "last_raw_blink", "last_dil_inv", "raw", "dec"}
+        dt, shp = spec.get(name, ((np.uint8, (h, w)) if name in u8 else (np.float32, (h, w))))
+        out = np.empty(shp, dt)
+        rc = lib().lvbgs_get_state(self._h, name.encode(), out.ctypes.data, out.nbytes)
+        if rc < 0:
+            raise KeyError(name)
+        return out
+
+    def set_timing(self, stage):
+        self._check(lib().lvbgs_set_timing(self._h, (stage or "").encode()))
+
+    def take_timing(self):
+        ms, n = C.c_double(), C.c_int()
+        self._check(lib().lvbgs_take_timing(self._h, C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
+    def set_fixpoint_schedule(self, launches, blocks):
+        self._check(lib().lvbgs_set_fixpoint_schedule(self._h, int(launches), int(blocks)))
+
+    def debug_rng(self, frame, phase, event, n):
+        out = np.empty((n, 4), np.uint32)
+        self._check(lib().lvbgs_debug_rng(self._h, frame, phase, event, n, out.ctypes.data))
+        return out
+
+
+class BackgroundSubtractorSuBSENSE(BackgroundSubtractor):
+    """BackgroundSubtractorSuBSENSE (BackgroundSubtractorSuBSENSE.hpp:48-135); default learning rate 0."""
+    ALGO = SUBSENSE
+
+
+class BackgroundSubtractorLOBSTER(BackgroundSubtractor):
+    """BackgroundSubtractorLOBSTER (BackgroundSubtractorLOBSTER.hpp:132-155); default learning rate 16."""
+    ALGO = LOBSTER

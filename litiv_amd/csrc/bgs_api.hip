@@ -1,0 +1,680 @@
+// bgs_api.hip -- C ABI (include/litiv_bgs.h): per-stream context, device memory layout,
+// initialize / apply / getters. Host-side logic mirrors the reference's IIBackgroundSubtractor
+// (BackgroundSubtractionUtils.cpp) and the constructors/initialize of SuBSENSE and LOBSTER.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/litiv_bgs.h"
+#include "bgs_kernels.hpp"
+
+using namespace lvbgs;
+
+namespace {
+
+__global__ void k_fill_f32(float* p, size_t n, float v) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+struct Arena {
+    uint8_t* base = nullptr;
+    size_t size = 0, off = 0;
+    template <typename T>
+    T* take(size_t count) {
+        off = (off + 255) & ~(size_t)255;
+        T* p = reinterpret_cast<T*>(base + off);
+        off += count * sizeof(T);
+        return p;
+    }
+};
+
+uint8_t sat_u8_host(float v) {
+    const long r = lrintf(v);  // cvRound: round half to even
+    return (uint8_t)(r < 0 ? 0 : r > 255 ? 255 : r);
+}
+
+}  // namespace
+
+struct lvbgs_ctx {
+    int algo = 0;
+    lvbgs_params p{};
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    // geometry / state
+    bool initialized = false;
+    int W = 0, H = 0, C = 0;
+    size_t npx = 0, n_roi = 0, orig_roi = 0;
+    std::vector<uint8_t> roi;  // validated ROI (host copy)
+    int roi_w = 0, roi_h = 0;
+    // device
+    uint8_t* arena = nullptr;
+    size_t arena_size = 0;
+    View v{};
+    uint8_t* d_frame = nullptr;  // staging for host-pointer apply / initialize
+    uint8_t* d_mask = nullptr;
+    StepOptions opt{};
+    uint32_t apply_count = 0;
+    // timing
+    int timed_stage = 0;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pairs;
+    size_t ev_used = 0;
+};
+
+static int set_err(lvbgs_ctx* c, int code, const char* msg) {
+    if (c) c->err = msg;
+    return code;
+}
+
+static int hip_check(lvbgs_ctx* c, hipError_t e, const char* where) {
+    if (e == hipSuccess) return LVBGS_OK;
+    char buf[256];
+    std::snprintf(buf, sizeof(buf), "%s: %s", where, hipGetErrorString(e));
+    return set_err(c, LVBGS_ERR_HIP, buf);
+}
+
+extern "C" int lvbgs_default_params(int algo, lvbgs_params* out) {
+    if (!out || (algo != LVBGS_SUBSENSE && algo != LVBGS_LOBSTER)) return LVBGS_ERR_ARG;
+    std::memset(out, 0, sizeof(*out));
+    if (algo == LVBGS_SUBSENSE) {  // BackgroundSubtractorSuBSENSE.hpp:22-31
+        out->desc_dist_threshold = 3;
+        out->color_dist_threshold = 30;
+        out->bg_samples = 50;
+        out->required_bg_samples = 2;
+        out->samples_for_mv_avgs = 100;
+    } else {  // BackgroundSubtractorLOBSTER.hpp:22-31
+        out->desc_dist_threshold = 4;
+        out->color_dist_threshold = 30;
+        out->bg_samples = 35;
+        out->required_bg_samples = 2;
+    }
+    out->lbsp_threshold_offset = 0;
+    out->rel_lbsp_threshold = 0.333f;
+    out->median_blur_kernel = 9;
+    out->rng_seed = 0xB65Eu;
+    out->stream_id = 0;
+    out->device = 0;
+    return LVBGS_OK;
+}
+
+extern "C" int lvbgs_create(int algo, const lvbgs_params* params, lvbgs_ctx** out) {
+    if (!out) return LVBGS_ERR_ARG;
+    *out = nullptr;
+    lvbgs_params p;
+    if (lvbgs_default_params(algo, &p) != LVBGS_OK) return LVBGS_ERR_ARG;
+    if (params) p = *params;
+    // SUB:76-77, LOB:28-29, plus this build's limits (8-bit slot indices, median halo of 6)
+    if (p.bg_samples == 0 || p.bg_samples > 255 || p.required_bg_samples > p.bg_samples ||
+        (p.color_dist_threshold == 0 && p.desc_dist_threshold == 0) || !(p.rel_lbsp_threshold >= 0.f) ||
+        (algo == LVBGS_SUBSENSE && p.samples_for_mv_avgs < 4) || p.median_blur_kernel < 1 ||
+        p.median_blur_kernel > 13 || (p.median_blur_kernel % 2) == 0 || p.color_dist_threshold > 0xFFFFFF ||
+        p.desc_dist_threshold > 0xFFFFFF || p.lbsp_threshold_offset > 255)
+        return LVBGS_ERR_ARG;
+    lvbgs_ctx* c = new lvbgs_ctx();
+    c->algo = algo;
+    c->p = p;
+    c->device = p.device;
+    if (hipSetDevice(c->device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return LVBGS_ERR_HIP;
+    }
+    *out = c;
+    return LVBGS_OK;
+}
+
+static void release_device(lvbgs_ctx* c) {
+    if (c->arena) hipFree(c->arena);
+    c->arena = nullptr;
+    c->arena_size = 0;
+}
+
+extern "C" void lvbgs_destroy(lvbgs_ctx* c) {
+    if (!c) return;
+    hipSetDevice(c->device);
+    if (c->stream) hipStreamSynchronize(c->stream);
+    for (auto& e : c->ev_pairs) {
+        hipEventDestroy(e.first);
+        hipEventDestroy(e.second);
+    }
+    release_device(c);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+}
+
+extern "C" const char* lvbgs_last_error(const lvbgs_ctx* c) { return c ? c->err.c_str() : "null context"; }
+extern "C" void* lvbgs_stream(lvbgs_ctx* c) { return c ? (void*)c->stream : nullptr; }
+extern "C" double lvbgs_default_learning_rate(const lvbgs_ctx* c) {
+    return (c && c->algo == LVBGS_LOBSTER) ? 16.0 : 0.0;
+}
+
+extern "C" int lvbgs_validate_roi(int w, int h, uint8_t* roi, size_t stride) {
+    if (!roi || w <= 0 || h <= 0 || stride < (size_t)w) return LVBGS_ERR_ARG;
+    for (int y = 0; y < h; ++y)
+        for (int x = 0; x < w; ++x)
+            if (x < 2 || y < 2 || x >= w - 2 || y >= h - 2) roi[(size_t)y * stride + x] = 0;
+    return LVBGS_OK;
+}
+
+// allocates (or re-uses) the device arena for the current geometry and carves the View
+static int allocate(lvbgs_ctx* c) {
+    const size_t npx = c->npx, C = (size_t)c->C, N = c->p.bg_samples;
+    const bool sub = c->algo == LVBGS_SUBSENSE;
+    const int dsW = c->W / 8, dsH = c->H / 8;
+    const size_t nds = (size_t)(dsW > 0 ? dsW : 1) * (size_t)(dsH > 0 ? dsH : 1) * C;
+    // dry run to size the arena
+    Arena a;
+    View& v = c->v;
+    auto carve = [&](Arena& ar) {
+        v.roi = ar.take<uint8_t>(npx);
+        v.last_color = ar.take<uint8_t>(npx * C);
+        v.last_desc = ar.take<uint16_t>(npx * C);
+        v.sA = ar.take<uint8_t>(N * npx * (C == 3 ? 8 : 2));
+        v.sB = ar.take<uint8_t>(N * npx);
+        float** maps[] = {&v.T, &v.R, &v.V, &v.MLD, &v.DminLT, &v.DminST, &v.RawLT, &v.RawST, &v.FinLT, &v.FinST};
+        for (auto m : maps) *m = sub ? ar.take<float>(npx) : nullptr;
+        uint8_t** u8s[] = {&v.unstable, &v.blinks, &v.last_raw, &v.last_raw_blink, &v.last_fg, &v.last_dil_inv,
+                           &v.raw, &v.pre, &v.mh, &v.dec};
+        for (auto m : u8s) *m = ar.take<uint8_t>(npx);
+        v.label = ar.take<uint32_t>(npx);
+        v.desc_cur = ar.take<uint16_t>(npx * C);
+        v.rec = ar.take<uint32_t>(npx);
+        v.mres = ar.take<uint32_t>(npx);
+        v.wl0 = ar.take<uint32_t>(npx);
+        v.wl1 = ar.take<uint32_t>(npx);
+        v.stamp = ar.take<unsigned long long>(npx);
+        v.dsLT = ar.take<float>(nds);
+        v.dsST = ar.take<float>(nds);
+        v.sc = ar.take<Scalars>(1);
+        c->d_frame = ar.take<uint8_t>(npx * C);
+        c->d_mask = ar.take<uint8_t>(npx);
+    };
+    a.base = nullptr;
+    carve(a);
+    const size_t need = a.off + 256;
+    if (need != c->arena_size) {
+        release_device(c);
+        if (hipMalloc(&c->arena, need) != hipSuccess) {
+            c->arena = nullptr;
+            return set_err(c, LVBGS_ERR_ALLOC, "device allocation failed for the background model");
+        }
+        c->arena_size = need;
+    }
+    Arena b;
+    b.base = c->arena;
+    carve(b);
+    return hipMemsetAsync(c->arena, 0, c->arena_size, c->stream) == hipSuccess ? LVBGS_OK
+                                                                               : set_err(c, LVBGS_ERR_HIP, "memset");
+}
+
+static int upload_scalars(lvbgs_ctx* c, const Scalars& s) {
+    return hip_check(c, hipMemcpyAsync(c->v.sc, &s, sizeof(Scalars), hipMemcpyHostToDevice, c->stream), "scalars");
+}
+
+static int download_scalars(lvbgs_ctx* c, Scalars& s) {
+    int rc = hip_check(c, hipMemcpyAsync(&s, c->v.sc, sizeof(Scalars), hipMemcpyDeviceToHost, c->stream), "scalars");
+    if (rc) return rc;
+    return hip_check(c, hipStreamSynchronize(c->stream), "sync");
+}
+
+extern "C" int lvbgs_initialize(lvbgs_ctx* c, const uint8_t* img, int w, int h, int ch, size_t stride,
+                                const uint8_t* roi, size_t roi_stride) {
+    if (!c) return LVBGS_ERR_ARG;
+    c->err.clear();
+    if (!img || w <= 0 || h <= 0) return set_err(c, LVBGS_ERR_ARG, "provided image for initialization must be non-empty");
+    if (ch != 1 && ch != 3)
+        return set_err(c, LVBGS_ERR_ARG, "provided image for initialization must be of type 8UC1/8UC3");
+    if (stride == 0) stride = (size_t)w * ch;
+    if (stride < (size_t)w * ch) return set_err(c, LVBGS_ERR_ARG, "image stride too small");
+    if ((size_t)w * (size_t)h > 0x7FFFFFFFull / 8) return set_err(c, LVBGS_ERR_ARG, "image too large");
+    if (hipSetDevice(c->device) != hipSuccess) return set_err(c, LVBGS_ERR_HIP, "hipSetDevice");
+    const size_t npx = (size_t)w * h;
+    // ROI (BackgroundSubtractionUtils.cpp:82-101)
+    std::vector<uint8_t> nr(npx);
+    if (!roi) {
+        if (!c->roi.empty() && c->roi_w == w && c->roi_h == h) nr = c->roi;  // reuse (:87-88)
+        else std::memset(nr.data(), 255, npx);
+    } else {
+        if (roi_stride == 0) roi_stride = (size_t)w;
+        if (roi_stride < (size_t)w) return set_err(c, LVBGS_ERR_ARG, "ROI stride too small");
+        for (int y = 0; y < h; ++y)
+            for (int x = 0; x < w; ++x) {
+                const uint8_t r = roi[(size_t)y * roi_stride + x];
+                if (r != 0 && r != 255)
+                    return set_err(c, LVBGS_ERR_ARG, "provided ROI mat values must be 0 or 255 only");
+                nr[(size_t)y * w + x] = r;
+            }
+        // roi | dilate(roi, 2 iterations)/2 (:92-95): the 5x5 dilation marks nearby pixels (value 128)
+        std::vector<uint8_t> dil(npx, 0);
+        for (int y = 0; y < h; ++y)
+            for (int x = 0; x < w; ++x) {
+                uint8_t m = 0;
+                for (int yy = y - 2; yy <= y + 2 && !m; ++yy)
+                    for (int xx = x - 2; xx <= x + 2; ++xx)
+                        if (yy >= 0 && yy < h && xx >= 0 && xx < w && nr[(size_t)yy * w + xx]) {
+                            m = 128;
+                            break;
+                        }
+                dil[(size_t)y * w + x] = m;
+            }
+        for (size_t i = 0; i < npx; ++i) nr[i] = nr[i] | dil[i];
+    }
+    size_t orig = 0;
+    for (size_t i = 0; i < npx; ++i) orig += nr[i] ? 1 : 0;
+    if (!orig) return set_err(c, LVBGS_ERR_ARG, "provided ROI mat contains no useful pixels");
+    lvbgs_validate_roi(w, h, nr.data(), (size_t)w);
+    size_t fin = 0;
+    for (size_t i = 0; i < npx; ++i) fin += nr[i] ? 1 : 0;
+    if (!fin) return set_err(c, LVBGS_ERR_ARG, "provided ROI mat contains no useful pixels away from borders");
+
+    c->initialized = false;
+    c->W = w;
+    c->H = h;
+    c->C = ch;
+    c->npx = npx;
+    c->orig_roi = orig;
+    c->n_roi = fin;
+    c->roi = nr;
+    c->roi_w = w;
+    c->roi_h = h;
+    c->apply_count = 0;
+    int rc = allocate(c);
+    if (rc) return rc;
+    View& v = c->v;
+    v.W = w;
+    v.H = h;
+    v.C = ch;
+    v.N = (int)c->p.bg_samples;
+    v.REQ = (int)c->p.required_bg_samples;
+    v.algo = c->algo;
+    v.npx = (uint32_t)npx;
+    v.n_roi = (uint32_t)fin;
+    v.color_thr = (uint32_t)c->p.color_dist_threshold;
+    v.desc_thr = (uint32_t)c->p.desc_dist_threshold;
+    v.mv_avgs = (uint32_t)c->p.samples_for_mv_avgs;
+    v.lbsp_offset = (uint32_t)c->p.lbsp_threshold_offset;
+    v.rel_lbsp = c->p.rel_lbsp_threshold;
+    v.seed_lo = (uint32_t)c->p.rng_seed;
+    v.seed_hi = (uint32_t)(c->p.rng_seed >> 32);
+    v.stream_id = c->p.stream_id;
+    v.dsW = w / 8;
+    v.dsH = h / 8;
+
+    Scalars s;
+    std::memset(&s, 0, sizeof(s));
+    // LBSP LUT (BackgroundSubtractorLBSP.cpp:29-30, :42-43)
+    for (int t = 0; t <= 255; ++t) {
+        const float base = (float)t * c->p.rel_lbsp_threshold + (float)c->p.lbsp_threshold_offset;
+        s.lut[t] = sat_u8_host(ch == 1 ? base / 3.0f : base);
+    }
+    float lower = 2.0f, upper = 256.0f;
+    if (c->algo == LVBGS_SUBSENSE) {
+        // SuBSENSE .cpp:112-128
+        if (orig >= npx / 2 && (int)npx >= 320 * 240) {
+            v.lr_scaling = 1;
+            s.auto_reset = 1;
+            v.use3x3 = !((int)npx > 320 * 240 * 2);
+            const int raw = (int)std::floor((float)npx / (float)(320 * 240) + 0.5f) + c->p.median_blur_kernel;
+            const int rk = raw < 14 ? raw : 14;
+            v.median_k = (rk % 2) ? rk : rk - 1;
+        } else {
+            v.lr_scaling = 0;
+            s.auto_reset = 0;
+            v.use3x3 = 1;
+            v.median_k = c->p.median_blur_kernel;
+            lower = 4.0f;
+            upper = 512.0f;
+        }
+    } else {
+        v.lr_scaling = 0;
+        v.use3x3 = 1;
+        v.median_k = c->p.median_blur_kernel;
+    }
+    s.lower_cap = lower;
+    s.upper_cap = upper;
+    if ((rc = upload_scalars(c, s))) return rc;
+    if ((rc = hip_check(c, hipMemcpyAsync((void*)v.roi, nr.data(), npx, hipMemcpyHostToDevice, c->stream), "roi"))) return rc;
+    if ((rc = hip_check(c, hipMemcpy2DAsync(c->d_frame, (size_t)w * ch, img, stride, (size_t)w * ch, (size_t)h,
+                                            hipMemcpyHostToDevice, c->stream), "frame")))
+        return rc;
+    if (c->algo == LVBGS_SUBSENSE) {
+        const unsigned b = (unsigned)((npx + 255) / 256);
+        hipLaunchKernelGGL(k_fill_f32, dim3(b), dim3(256), 0, c->stream, v.T, npx, lower);
+        hipLaunchKernelGGL(k_fill_f32, dim3(b), dim3(256), 0, c->stream, v.R, npx, 1.0f);
+        hipLaunchKernelGGL(k_fill_f32, dim3(b), dim3(256), 0, c->stream, v.V, npx, 10.0f);
+    }
+    launch_init_desc(v, c->d_frame, c->stream);
+    // refreshModel(1.0f) for SuBSENSE (.cpp:184), refreshModel(1.0f, true) for LOBSTER (.cpp:455)
+    launch_refresh(v, 1.0f, c->algo == LVBGS_LOBSTER ? 1 : 0, 0u, 0u, c->stream);
+    s.refresh_seq = 1;
+    // the refresh consumed event 0; store the counter
+    if ((rc = hip_check(c, hipMemcpyAsync(&v.sc->refresh_seq, &s.refresh_seq, sizeof(uint32_t), hipMemcpyHostToDevice,
+                                          c->stream), "scalars")))
+        return rc;
+    if ((rc = hip_check(c, hipGetLastError(), "initialize launch"))) return rc;
+    if ((rc = hip_check(c, hipStreamSynchronize(c->stream), "initialize"))) return rc;
+    c->initialized = true;
+    return LVBGS_OK;
+}
+
+static int check_lr(lvbgs_ctx* c, double lr) {
+    if (c->algo == LVBGS_LOBSTER && !(lr > 0))
+        return set_err(c, LVBGS_ERR_ARG, "learning rate must be a positive value; faster learning is achieved with smaller values");
+    return LVBGS_OK;
+}
+
+static void step(lvbgs_ctx* c, const uint8_t* d_img, uint8_t* d_mask, double lr) {
+    StepOptions o = c->opt;
+    o.timed_stage = 0;
+    if (c->timed_stage) {
+        if (c->ev_used == c->ev_pairs.size()) {
+            hipEvent_t a, b;
+            hipEventCreate(&a);
+            hipEventCreate(&b);
+            c->ev_pairs.emplace_back(a, b);
+        }
+        o.timed_stage = c->timed_stage;
+        o.ev_begin = c->ev_pairs[c->ev_used].first;
+        o.ev_end = c->ev_pairs[c->ev_used].second;
+        ++c->ev_used;
+    }
+    frame_step(c->v, d_img, d_mask, lr, o, c->stream);
+    ++c->apply_count;
+}
+
+extern "C" int lvbgs_apply(lvbgs_ctx* c, const uint8_t* img, size_t stride, uint8_t* fgmask, size_t mask_stride,
+                           double lr) {
+    if (!c) return LVBGS_ERR_ARG;
+    if (!c->initialized) return set_err(c, LVBGS_ERR_STATE, "algo & model must be initialized first");
+    if (!img) return set_err(c, LVBGS_ERR_ARG, "input image type/size mismatch with initialization type/size");
+    int rc;
+    if ((rc = check_lr(c, lr))) return rc;
+    hipSetDevice(c->device);
+    const size_t row = (size_t)c->W * c->C;
+    if (stride == 0) stride = row;
+    if (mask_stride == 0) mask_stride = (size_t)c->W;
+    if (stride < row || (fgmask && mask_stride < (size_t)c->W)) return set_err(c, LVBGS_ERR_ARG, "stride too small");
+    if ((rc = hip_check(c, hipMemcpy2DAsync(c->d_frame, row, img, stride, row, (size_t)c->H, hipMemcpyHostToDevice,
+                                            c->stream), "input copy")))
+        return rc;
+    step(c, c->d_frame, c->d_mask, lr);
+    if ((rc = hip_check(c, hipGetLastError(), "apply launch"))) return rc;
+    if (fgmask &&
+        (rc = hip_check(c, hipMemcpy2DAsync(fgmask, mask_stride, c->d_mask, (size_t)c->W, (size_t)c->W, (size_t)c->H,
+                                            hipMemcpyDeviceToHost, c->stream), "mask copy")))
+        return rc;
+    return lvbgs_synchronize(c);
+}
+
+extern "C" int lvbgs_apply_device(lvbgs_ctx* c, const void* d_img, void* d_mask, double lr) {
+    if (!c) return LVBGS_ERR_ARG;
+    if (!c->initialized) return set_err(c, LVBGS_ERR_STATE, "algo & model must be initialized first");
+    if (!d_img) return set_err(c, LVBGS_ERR_ARG, "input image must be a device pointer");
+    int rc;
+    if ((rc = check_lr(c, lr))) return rc;
+    hipSetDevice(c->device);
+    step(c, (const uint8_t*)d_img, (uint8_t*)d_mask, lr);
+    return hip_check(c, hipGetLastError(), "apply launch");
+}
+
+extern "C" int lvbgs_synchronize(lvbgs_ctx* c) {
+    if (!c) return LVBGS_ERR_ARG;
+    hipSetDevice(c->device);
+    int rc = hip_check(c, hipStreamSynchronize(c->stream), "synchronize");
+    if (rc || !c->initialized) return rc;
+    uint32_t flags = 0;
+    rc = hip_check(c, hipMemcpy(&flags, &c->v.sc->err_flags, sizeof(flags), hipMemcpyDeviceToHost), "error flags");
+    if (rc) return rc;
+    if (flags & kErrFixpointCap) return set_err(c, LVBGS_ERR_DEVICE, "fix-point iteration cap reached");
+    return LVBGS_OK;
+}
+
+extern "C" int lvbgs_get_background_image(lvbgs_ctx* c, uint8_t* out, size_t stride) {
+    if (!c || !out) return LVBGS_ERR_ARG;
+    if (!c->initialized) return set_err(c, LVBGS_ERR_STATE, "algo must be initialized first");
+    hipSetDevice(c->device);
+    const size_t row = (size_t)c->W * c->C;
+    if (stride == 0) stride = row;
+    if (stride < row) return set_err(c, LVBGS_ERR_ARG, "stride too small");
+    launch_background(c->v, c->d_frame, nullptr, c->stream);
+    int rc = hip_check(c, hipMemcpy2DAsync(out, stride, c->d_frame, row, row, (size_t)c->H, hipMemcpyDeviceToHost,
+                                           c->stream), "background copy");
+    if (rc) return rc;
+    return hip_check(c, hipStreamSynchronize(c->stream), "sync");
+}
+
+extern "C" int lvbgs_get_background_descriptors_image(lvbgs_ctx* c, uint16_t* out, size_t stride) {
+    if (!c || !out) return LVBGS_ERR_ARG;
+    if (!c->initialized) return set_err(c, LVBGS_ERR_STATE, "algo must be initialized first");
+    hipSetDevice(c->device);
+    const size_t row = (size_t)c->W * c->C * 2;
+    if (stride == 0) stride = row;
+    if (stride < row) return set_err(c, LVBGS_ERR_ARG, "stride too small");
+    // staging: the per-step descriptor scratch (same size, only live inside a step)
+    uint16_t* tmp = c->v.desc_cur;
+    launch_background(c->v, nullptr, tmp, c->stream);
+    int rc = hip_check(c, hipMemcpy2DAsync(out, stride, tmp, row, row, (size_t)c->H, hipMemcpyDeviceToHost, c->stream),
+                       "descriptor copy");
+    if (rc) return rc;
+    return hip_check(c, hipStreamSynchronize(c->stream), "sync");
+}
+
+extern "C" int lvbgs_set_automatic_model_reset(lvbgs_ctx* c, int enabled) {
+    if (!c) return LVBGS_ERR_ARG;
+    if (!c->initialized) return LVBGS_OK;  // initialize() sets the flag itself (SUB:113-123)
+    hipSetDevice(c->device);
+    const int32_t val = enabled ? 1 : 0;
+    int rc = hip_check(c, hipMemcpyAsync(&c->v.sc->auto_reset, &val, sizeof(val), hipMemcpyHostToDevice, c->stream),
+                       "auto reset");
+    if (rc) return rc;
+    return hip_check(c, hipStreamSynchronize(c->stream), "sync");
+}
+
+extern "C" int lvbgs_refresh_model(lvbgs_ctx* c, float frac, int force_fg) {
+    if (!c) return LVBGS_ERR_ARG;
+    if (!c->initialized) return set_err(c, LVBGS_ERR_STATE, "algo must be initialized first");
+    if (!(frac > 0.0f && frac <= 1.0f)) return set_err(c, LVBGS_ERR_ARG, "model refresh must be given as a non-null fraction");
+    hipSetDevice(c->device);
+    Scalars s;
+    int rc = download_scalars(c, s);
+    if (rc) return rc;
+    const uint32_t ev = s.refresh_seq;
+    launch_refresh(c->v, frac, force_fg ? 1 : 0, s.frame_idx, ev, c->stream);
+    const uint32_t next = ev + 1;
+    if ((rc = hip_check(c, hipMemcpyAsync(&c->v.sc->refresh_seq, &next, sizeof(next), hipMemcpyHostToDevice, c->stream),
+                        "refresh counter")))
+        return rc;
+    return hip_check(c, hipStreamSynchronize(c->stream), "sync");
+}
+
+extern "C" int lvbgs_get_roi(lvbgs_ctx* c, uint8_t* out, size_t stride) {
+    if (!c || !out || c->roi.empty()) return LVBGS_ERR_ARG;
+    if (stride == 0) stride = (size_t)c->roi_w;
+    for (int y = 0; y < c->roi_h; ++y) std::memcpy(out + (size_t)y * stride, c->roi.data() + (size_t)y * c->roi_w, c->roi_w);
+    return LVBGS_OK;
+}
+
+extern "C" int lvbgs_set_roi(lvbgs_ctx* c, uint8_t* roi, size_t stride) {
+    if (!c || !roi) return LVBGS_ERR_ARG;
+    const int w = c->initialized ? c->W : c->roi_w, h = c->initialized ? c->H : c->roi_h;
+    if (w <= 0 || h <= 0) return set_err(c, LVBGS_ERR_ARG, "setROI needs a known frame size (initialize first)");
+    if (stride == 0) stride = (size_t)w;
+    lvbgs_validate_roi(w, h, roi, stride);
+    size_t nz = 0;
+    for (int y = 0; y < h; ++y)
+        for (int x = 0; x < w; ++x) nz += roi[(size_t)y * stride + x] ? 1 : 0;
+    if (!nz) return set_err(c, LVBGS_ERR_ARG, "provided ROI must have at least one valid pixel");
+    if (c->initialized) {
+        std::vector<uint8_t> bg((size_t)c->W * c->H * c->C);
+        int rc = lvbgs_get_background_image(c, bg.data(), 0);
+        if (rc) return rc;
+        return lvbgs_initialize(c, bg.data(), c->W, c->H, c->C, 0, roi, stride);
+    }
+    c->roi.assign((size_t)w * h, 0);
+    for (int y = 0; y < h; ++y) std::memcpy(c->roi.data() + (size_t)y * w, roi + (size_t)y * stride, w);
+    return LVBGS_OK;
+}
+
+extern "C" int lvbgs_lbsp_compute_abs(const void* d_img, int w, int h, int ch, int threshold, void* d_out, void* stream) {
+    if (!d_img || !d_out || w <= 0 || h <= 0 || (ch != 1 && ch != 3)) return LVBGS_ERR_ARG;
+    const uint32_t t = (uint32_t)(threshold < 0 ? 0 : threshold > 255 ? 255 : threshold);  // saturate_cast<uchar>
+    launch_lbsp_abs((const uint8_t*)d_img, w, h, ch, t, (uint16_t*)d_out, (hipStream_t)stream);
+    return hipGetLastError() == hipSuccess ? LVBGS_OK : LVBGS_ERR_HIP;
+}
+
+extern "C" int lvbgs_get_stats(lvbgs_ctx* c, lvbgs_stats* out) {
+    if (!c || !out) return LVBGS_ERR_ARG;
+    if (!c->initialized) return set_err(c, LVBGS_ERR_STATE, "algo must be initialized first");
+    Scalars s;
+    int rc = download_scalars(c, s);
+    if (rc) return rc;
+    out->examined = s.cnt_examined;
+    out->writes = s.cnt_writes;
+    out->fixpoint_iters = s.iters;
+    out->max_fixpoint_iter = s.max_iter_seen;
+    out->width = (uint32_t)c->W;
+    out->height = (uint32_t)c->H;
+    out->channels = (uint32_t)c->C;
+    out->n_roi = (uint32_t)c->n_roi;
+    return LVBGS_OK;
+}
+
+extern "C" int lvbgs_get_scalars(lvbgs_ctx* c, double* out, int cap) {
+    if (!c || !out) return LVBGS_ERR_ARG;
+    if (!c->initialized) return set_err(c, LVBGS_ERR_STATE, "algo must be initialized first");
+    Scalars s;
+    int rc = download_scalars(c, s);
+    if (rc) return rc;
+    const double vals[] = {(double)s.frame_idx, (double)s.frames_since_reset, (double)s.cooldown, (double)s.auto_reset,
+                           (double)c->v.lr_scaling, (double)c->v.use3x3, (double)c->v.median_k, (double)c->n_roi,
+                           (double)s.lower_cap, (double)s.upper_cap, (double)s.last_nz_ratio, (double)s.refresh_seq,
+                           (double)c->apply_count};
+    const int n = (int)(sizeof(vals) / sizeof(vals[0]));
+    for (int i = 0; i < n && i < cap; ++i) out[i] = vals[i];
+    return n;
+}
+
+extern "C" long long lvbgs_get_state(lvbgs_ctx* c, const char* name, void* out, size_t cap) {
+    if (!c || !name || !out) return LVBGS_ERR_ARG;
+    if (!c->initialized) return LVBGS_ERR_STATE;
+    hipSetDevice(c->device);
+    const View& v = c->v;
+    const size_t npx = c->npx, C = (size_t)c->C, N = c->p.bg_samples;
+    const std::string n(name);
+    const void* src = nullptr;
+    size_t bytes = 0;
+    std::vector<uint8_t> unpack;
+    if (n == "samples_color" || n == "samples_desc") {
+        // unpack the sample planes to the oracle layout: N x (H x W x C)
+        std::vector<uint8_t> a(N * npx * (C == 3 ? 8 : 2)), b(N * npx);
+        if (hipMemcpy(a.data(), v.sA, a.size(), hipMemcpyDeviceToHost) != hipSuccess ||
+            hipMemcpy(b.data(), v.sB, b.size(), hipMemcpyDeviceToHost) != hipSuccess)
+            return LVBGS_ERR_HIP;
+        const bool col = n == "samples_color";
+        bytes = N * npx * C * (col ? 1 : 2);
+        if (bytes > cap) return LVBGS_ERR_ARG;
+        for (size_t i = 0; i < N * npx; ++i) {
+            if (C == 3) {
+                uint64_t w;
+                std::memcpy(&w, a.data() + i * 8, 8);
+                if (col) {
+                    uint8_t* o = (uint8_t*)out + i * 3;
+                    o[0] = (uint8_t)(w >> 48);
+                    o[1] = (uint8_t)(w >> 56);
+                    o[2] = b[i];
+                } else {
+                    uint16_t* o = (uint16_t*)out + i * 3;
+                    o[0] = (uint16_t)w;
+                    o[1] = (uint16_t)(w >> 16);
+                    o[2] = (uint16_t)(w >> 32);
+                }
+            } else {
+                if (col) ((uint8_t*)out)[i] = b[i];
+                else std::memcpy((uint16_t*)out + i, a.data() + i * 2, 2);
+            }
+        }
+        return (long long)bytes;
+    }
+    struct Item {
+        const char* name;
+        const void* ptr;
+        size_t bytes;
+    };
+    const size_t fb = npx * 4, nds = (size_t)v.dsW * v.dsH * C * 4;
+    const Item items[] = {{"last_color", v.last_color, npx * C},  {"last_desc", v.last_desc, npx * C * 2},
+                          {"last_fg", v.last_fg, npx},             {"roi", v.roi, npx},
+                          {"lut", v.sc->lut, 256},                 {"T", v.T, fb},
+                          {"R", v.R, fb},                          {"V", v.V, fb},
+                          {"MLD", v.MLD, fb},                      {"DminLT", v.DminLT, fb},
+                          {"DminST", v.DminST, fb},                {"RawLT", v.RawLT, fb},
+                          {"RawST", v.RawST, fb},                  {"FinLT", v.FinLT, fb},
+                          {"FinST", v.FinST, fb},                  {"unstable", v.unstable, npx},
+                          {"blinks", v.blinks, npx},               {"last_raw", v.last_raw, npx},
+                          {"last_raw_blink", v.last_raw_blink, npx}, {"last_dil_inv", v.last_dil_inv, npx},
+                          {"dsLT", v.dsLT, nds},                   {"dsST", v.dsST, nds},
+                          {"raw", v.raw, npx},                     {"dec", v.dec, npx},
+                          {"mres", v.mres, npx * 4},               {"rec", v.rec, npx * 4},
+                          {"desc_cur", v.desc_cur, npx * C * 2}};
+    for (const auto& it : items)
+        if (n == it.name) {
+            src = it.ptr;
+            bytes = it.bytes;
+        }
+    if (!src) return LVBGS_ERR_ARG;
+    if (bytes > cap) return LVBGS_ERR_ARG;
+    if (hipStreamSynchronize(c->stream) != hipSuccess) return LVBGS_ERR_HIP;
+    if (hipMemcpy(out, src, bytes, hipMemcpyDeviceToHost) != hipSuccess) return LVBGS_ERR_HIP;
+    return (long long)bytes;
+}
+
+extern "C" int lvbgs_set_timing(lvbgs_ctx* c, const char* stage) {
+    if (!c) return LVBGS_ERR_ARG;
+    const std::string s(stage ? stage : "");
+    if (s.empty()) c->timed_stage = 0;
+    else if (s == "match") c->timed_stage = 1;
+    else if (s == "step") c->timed_stage = 2;
+    else return set_err(c, LVBGS_ERR_ARG, "unknown timing stage");
+    c->ev_used = 0;
+    return LVBGS_OK;
+}
+
+extern "C" int lvbgs_take_timing(lvbgs_ctx* c, double* total_ms, int* count) {
+    if (!c || !total_ms || !count) return LVBGS_ERR_ARG;
+    int rc = hip_check(c, hipStreamSynchronize(c->stream), "sync");
+    if (rc) return rc;
+    double tot = 0.0;
+    for (size_t i = 0; i < c->ev_used; ++i) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, c->ev_pairs[i].first, c->ev_pairs[i].second) != hipSuccess)
+            return set_err(c, LVBGS_ERR_HIP, "event timing");
+        tot += ms;
+    }
+    *total_ms = tot;
+    *count = (int)c->ev_used;
+    c->ev_used = 0;
+    return LVBGS_OK;
+}
+
+extern "C" int lvbgs_set_fixpoint_schedule(lvbgs_ctx* c, int launches, int blocks) {
+    if (!c || launches < 0 || launches > 64 || blocks < 1 || blocks > 65535) return LVBGS_ERR_ARG;
+    c->opt.fixpoint_launches = launches;
+    c->opt.fixpoint_blocks = blocks;
+    return LVBGS_OK;
+}
+
+// RNG self-check for tests: draws of group 0 for pixels [0,n) of a (frame, phase, event) counter
+extern "C" int lvbgs_debug_rng(lvbgs_ctx* c, uint32_t frame, uint32_t phase, uint32_t ev, uint32_t n, uint32_t* host_out) {
+    if (!c || !host_out || !c->initialized || n == 0) return LVBGS_ERR_ARG;
+    hipSetDevice(c->device);
+    uint32_t* d = nullptr;
+    if (hipMalloc(&d, (size_t)n * 16) != hipSuccess) return LVBGS_ERR_ALLOC;
+    launch_rng_probe(c->v, frame, phase, ev, n, d, c->stream);
+    int rc = hip_check(c, hipMemcpyAsync(host_out, d, (size_t)n * 16, hipMemcpyDeviceToHost, c->stream), "rng copy");
+    if (!rc) rc = hip_check(c, hipStreamSynchronize(c->stream), "sync");
+    hipFree(d);
+    return rc;
+}

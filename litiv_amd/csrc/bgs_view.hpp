@@ -1,0 +1,47 @@
+// bgs_view.hpp -- per-stream device state of a background subtractor, passed by value to every
+// kernel. All arrays are full-frame, row-major, pixel index p = y*W + x.
+#pragma once
+#include <stdint.h>
+
+#include "bgs_common.hpp"
+
+namespace lvbgs {
+
+enum Algo : int { kSuBSENSE = 1, kLOBSTER = 2 };
+
+struct View {
+    int W, H, C, N, REQ, algo;
+    uint32_t npx;            // W*H
+    uint32_t n_roi;          // number of ROI pixels
+    uint32_t color_thr;      // SuBSENSE m_nMinColorDistThreshold | LOBSTER m_nColorDistThreshold
+    uint32_t desc_thr;       // SuBSENSE m_nDescDistThresholdOffset | LOBSTER m_nDescDistThreshold
+    uint32_t mv_avgs;        // SuBSENSE m_nSamplesForMovingAvgs
+    uint32_t lbsp_offset;    // m_nLBSPThresholdOffset
+    float rel_lbsp;          // m_fRelLBSPThreshold
+    uint32_t seed_lo, seed_hi, stream_id;
+    int use3x3, median_k, lr_scaling, dsW, dsH;
+    // roi and model
+    const uint8_t* roi;      // W*H, 0 = outside
+    uint8_t* last_color;     // W*H*C
+    uint16_t* last_desc;     // W*H*C
+    void* sA;                // N sample planes: C==3 -> u64 {d0,d1,d2,c0,c1}; C==1 -> u16 {d0}
+    uint8_t* sB;             // N sample planes: C==3 -> u8 {c2}; C==1 -> u8 {c0}
+    // SuBSENSE feedback maps (f32 W*H)
+    float *T, *R, *V, *MLD, *DminLT, *DminST, *RawLT, *RawST, *FinLT, *FinST;
+    // u8 W*H maps
+    uint8_t *unstable, *blinks, *last_raw, *last_raw_blink, *last_fg, *last_dil_inv;
+    // per-frame scratch
+    uint8_t *raw, *pre, *mh;
+    uint32_t* label;
+    uint16_t* desc_cur;      // W*H*C intra descriptors of the current frame
+    uint32_t* rec;           // W*H update records
+    uint32_t* mres;          // W*H match results
+    uint8_t* dec;            // W*H raw decisions (1 = foreground)
+    uint32_t* wl0;           // worklists (n_roi entries each)
+    uint32_t* wl1;
+    unsigned long long* stamp;  // W*H worklist de-duplication stamps
+    float *dsLT, *dsST;      // (W/8)*(H/8)*C
+    Scalars* sc;
+};
+
+}  // namespace lvbgs
