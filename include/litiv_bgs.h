@@ -113,6 +113,7 @@ typedef struct lvbgs_stats {
     uint32_t fixpoint_iters;  /* fix-point iterations that had work in the last step */
     uint32_t max_fixpoint_iter;
     uint32_t width, height, channels, n_roi;
+    uint64_t examined_speculative; /* samples examined by the speculative match pass alone */
 } lvbgs_stats;
 int lvbgs_get_stats(lvbgs_ctx* ctx, lvbgs_stats* out);
 

@@ -532,8 +532,14 @@ extern "C" int lvbgs_get_stats(lvbgs_ctx* c, lvbgs_stats* out) {
     Scalars s;
     int rc = download_scalars(c, s);
     if (rc) return rc;
-    out->examined = s.cnt_examined;
-    out->writes = s.cnt_writes;
+    auto sum = [](const unsigned long long* a) {
+        unsigned long long t = 0;
+        for (int i = 0; i < kStripes; ++i) t += a[i];
+        return t;
+    };
+    out->examined = sum(s.ex_part);
+    out->writes = sum(s.wr_part);
+    out->examined_speculative = sum(s.spec_part);
     out->fixpoint_iters = s.iters;
     out->max_fixpoint_iter = s.max_iter_seen;
     out->width = (uint32_t)c->W;

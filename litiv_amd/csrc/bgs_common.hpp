@@ -123,6 +123,7 @@ LV_HD void neighbor_pos(uint32_t rnd, bool use3, int x, int y, int w, int h, int
 // step needs no host round trip).
 // ------------------------------------------------------------------------------------------
 constexpr int kWlRing = 4;
+constexpr int kStripes = 64;
 struct alignas(16) Scalars {
     uint32_t frame_idx;          // frames applied since initialize (SuBSENSE m_nFrameIdx)
     uint32_t frames_since_reset; // m_nFramesSinceLastReset
@@ -131,9 +132,9 @@ struct alignas(16) Scalars {
     float lower_cap, upper_cap;  // m_fCurrLearningRate{Lower,Upper}Cap
     float last_nz_ratio;         // m_fLastNonZeroDescRatio
     uint32_t refresh_seq;        // refresh events so far (RNG event id)
-    uint32_t nz_count;           // per-frame: pixels with >= 4 (1ch: 2) descriptor bits
+    uint32_t nz_count_unused;
     uint32_t err_flags;          // sticky error bits (see kErr*)
-    unsigned long long cd_sum;   // per-frame: frame-level colour difference sum
+    unsigned long long cd_sum_unused;
     uint32_t wl_count[kWlRing];  // fix-point worklist counts (ring)
     int32_t refresh_pending;     // refresh request consumed by the refresh kernel
     float refresh_frac;
@@ -141,8 +142,13 @@ struct alignas(16) Scalars {
     uint32_t refresh_frame;
     uint32_t refresh_ev;
     uint32_t refresh_start;
-    unsigned long long cnt_examined; // per-frame: sum over pixels of samples examined (k_p)
-    unsigned long long cnt_writes;   // per-frame: sample writes (w_p)
+    // per-frame sums, striped over kStripes counters (one atomic per workgroup into stripe
+    // blockIdx % kStripes) and summed by the consumer
+    unsigned long long nz_part[64];   // pixels with >= 4 (1ch: 2) descriptor bits (LUT adaptation)
+    unsigned long long cd_part[64];   // frame-level colour difference sum (model reset, caps)
+    unsigned long long ex_part[64];   // samples examined by the final match (k_p)
+    unsigned long long wr_part[64];   // sample writes (w_p)
+    unsigned long long spec_part[64]; // samples examined by the speculative match pass
     uint32_t iters;              // per-frame: fix-point iterations that had work
     uint32_t max_iter_seen;      // largest fix-point iteration index that had work (sticky)
     uint32_t epoch;              // steps since allocation (worklist de-duplication tags)

@@ -42,6 +42,29 @@ LV_D uint32_t absdiff(uint32_t a, uint32_t b) { return a > b ? a - b : b - a; }
 LV_D float fmin_ref(float a, float b) { return (b < a) ? b : a; }  // std::min
 LV_D float fmax_ref(float a, float b) { return (a < b) ? b : a; }  // std::max
 
+// Workgroup sum of one value per thread, added to stripe blockIdx % kStripes of a counter array.
+// Every thread of the workgroup must call it.
+template <int NT>
+LV_D void block_add(unsigned long long value, unsigned long long* stripes) {
+    __shared__ unsigned long long part[NT / 64];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) value += __shfl_down(value, off, 64);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = value;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long s = 0;
+#pragma unroll
+        for (int i = 0; i < NT / 64; ++i) s += part[i];
+        if (s) atomicAdd(&stripes[blockIdx.x % kStripes], s);
+    }
+}
+
+LV_HD unsigned long long sum_stripes(const unsigned long long* s) {
+    unsigned long long t = 0;
+    for (int i = 0; i < kStripes; ++i) t += s[i];
+    return t;
+}
+
 // frame number of the step in flight (m_nFrameIdx after its increment at the top of apply)
 LV_D uint32_t cur_frame(const View& v) { return v.sc->frame_idx + 1u; }
 
@@ -340,9 +363,12 @@ __global__ void __launch_bounds__(TPB) k_describe(View v, const uint8_t* __restr
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         // per-frame counters consumed by later kernels of this step
         for (int i = 0; i < kWlRing; ++i) v.sc->wl_count[i] = 0;
-        v.sc->cnt_examined = 0;
-        v.sc->cnt_writes = 0;
         v.sc->iters = 0;
+    }
+    if (blockIdx.x == 0 && threadIdx.x < kStripes) {
+        v.sc->ex_part[threadIdx.x] = 0;
+        v.sc->wr_part[threadIdx.x] = 0;
+        v.sc->spec_part[threadIdx.x] = 0;
     }
     load_frame_tile<C>(tile, frame, v.W, v.H, x0, y0);
     load_lut(lut, v.sc);
@@ -407,10 +433,7 @@ __global__ void __launch_bounds__(TPB) k_describe(View v, const uint8_t* __restr
             v.rec[p] = r;
         }
     }
-    if constexpr (ALGO == kSuBSENSE) {
-        const unsigned long long b = __ballot(nz);
-        if ((threadIdx.x & 63) == 0 && b) atomicAdd(&v.sc->nz_count, (uint32_t)__popcll(b));
-    }
+    if constexpr (ALGO == kSuBSENSE) block_add<TPB>(nz ? 1ull : 0ull, v.sc->nz_part);
 }
 
 // builds the match context of ROI pixel (x,y); taps from the LDS tile or from global memory
@@ -433,6 +456,10 @@ LV_D void build_ctx(const View& v, const uint8_t* frame, const FrameTile<C>* til
 // speculative match: raster-earlier writers assumed to decide as in the previous frame
 // ----------------------------------------------------------------------------------------------
 template <int C, int ALGO>
+LV_D uint32_t match_pixel(const View& v, const uint8_t* frame, const uint8_t* lut, const FrameTile<C>& tile, int lx,
+                          int ly, int x, int y, uint32_t p);
+
+template <int C, int ALGO>
 __global__ void __launch_bounds__(TPB) k_match(View v, const uint8_t* __restrict__ frame) {
     __shared__ FrameTile<C> tile;
     __shared__ uint8_t lut[256];
@@ -443,9 +470,15 @@ __global__ void __launch_bounds__(TPB) k_match(View v, const uint8_t* __restrict
     load_lut(lut, v.sc);
     __syncthreads();
     const int lx = threadIdx.x % TW, ly = threadIdx.x / TW, x = x0 + lx, y = y0 + ly;
-    if (x >= v.W || y >= v.H) return;
     const uint32_t p = (uint32_t)(y * v.W + x);
-    if (!v.roi[p]) return;
+    uint32_t examined = 0;
+    if (x < v.W && y < v.H && v.roi[p]) examined = match_pixel<C, ALGO>(v, frame, lut, tile, lx, ly, x, y, p);
+    block_add<TPB>(examined, v.sc->spec_part);
+}
+
+template <int C, int ALGO>
+LV_D uint32_t match_pixel(const View& v, const uint8_t* frame, const uint8_t* lut, const FrameTile<C>& tile, int lx,
+                          int ly, int x, int y, uint32_t p) {
     PxCtx<C> px;
     build_ctx<C, ALGO, true>(v, frame, &tile, lx, ly, x, y, px);
     const uint32_t cand = candidate_writers(v, x, y);
@@ -462,6 +495,7 @@ __global__ void __launch_bounds__(TPB) k_match(View v, const uint8_t* __restrict
     const uint8_t guess = v.last_raw[p] ? 1 : 0;
     uint32_t t;
     if (fg != guess && forward_target(v, v.rec[p], x, y, t)) wl_push(v, t, 1u, v.sc->epoch, v.wl1);
+    return mr >> 24;
 }
 
 // re-match of one worklisted pixel with the current decisions of its writers
@@ -644,11 +678,7 @@ __global__ void __launch_bounds__(TPB) k_feedback(View v, const uint8_t* __restr
         v.last_raw[p] = raw;
         v.raw[p] = raw;
     }
-    // workload counter (sum of k_p)
-    unsigned long long e = examined;
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) e += __shfl_down(e, off, 64);
-    if ((threadIdx.x & 63) == 0 && e) atomicAdd(&v.sc->cnt_examined, e);
+    block_add<TPB>(examined, v.sc->ex_part);  // workload counter (sum of k_p)
 }
 
 // LOBSTER: raw mask (LOB:496-497, :554-555) and previous-raw bookkeeping for the next speculation
@@ -664,10 +694,7 @@ __global__ void __launch_bounds__(TPB) k_lobster_raw(View v) {
         v.raw[p] = raw;
         v.last_raw[p] = raw;
     }
-    unsigned long long e = examined;
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) e += __shfl_down(e, off, 64);
-    if ((threadIdx.x & 63) == 0 && e) atomicAdd(&v.sc->cnt_examined, e);
+    block_add<TPB>(examined, v.sc->ex_part);
 }
 
 // ----------------------------------------------------------------------------------------------
@@ -721,10 +748,7 @@ __global__ void __launch_bounds__(TPB) k_update(View v, const uint8_t* __restric
         }
         nw = (uint32_t)n;
     }
-    unsigned long long e = nw;
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) e += __shfl_down(e, off, 64);
-    if ((threadIdx.x & 63) == 0 && e) atomicAdd(&v.sc->cnt_writes, e);
+    block_add<TPB>(nw, v.sc->wr_part);
 }
 
 // LOBSTER: last colour frame := input (LOB:580)
@@ -875,7 +899,14 @@ __global__ void __launch_bounds__(TPB) k_ccl_border(View v) {
     if (ax >= v.W || ay >= v.H || bx >= v.W || by >= v.H) return;
     const uint32_t a = (uint32_t)(ay * v.W + ax), b = (uint32_t)(by * v.W + bx);
     if (v.pre[a] != 0 || v.pre[b] != 0) return;
-    g_union(v.label, a, b);
+    // labels still hold tile-local roots; the previous pixel along the edge usually carries the
+    // same pair of local roots, in which case its union already covers this one
+    const uint32_t la = v.label[a], lb = v.label[b];
+    if (k % CT != 0) {
+        const uint32_t pa = k < CT ? a - (uint32_t)v.W : a - 1u, pb = k < CT ? b - (uint32_t)v.W : b - 1u;
+        if (v.pre[pa] == 0 && v.pre[pb] == 0 && v.label[pa] == la && v.label[pb] == lb) return;
+    }
+    g_union(v.label, la, lb);
 }
 
 // holes = 0-pixels of PreFlood not connected to pixel (0,0) (floodFill + bitwise_not, SUB:541-543);
@@ -1052,21 +1083,24 @@ __global__ void __launch_bounds__(TPB) k_downsample(View v, const uint8_t* __res
         }
         contrib = C == 1 ? best / 2u : best;
     }
-    unsigned long long e = contrib;
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) e += __shfl_down(e, off, 64);
-    if ((threadIdx.x & 63) == 0 && e) atomicAdd(&v.sc->cd_sum, e);
+    block_add<TPB>(contrib, v.sc->cd_part);
 }
 
 // single-workgroup decision: LUT adaptation, model reset, learning-rate caps, counters
 __global__ void __launch_bounds__(256) k_decide(View v) {
     Scalars* sc = v.sc;
     __shared__ float s_ratio, s_last;
+    __shared__ unsigned long long s_cd;
     if (threadIdx.x == 0) {
-        s_ratio = (float)sc->nz_count / (float)v.n_roi;
+        s_ratio = (float)sum_stripes(sc->nz_part) / (float)v.n_roi;
         s_last = sc->last_nz_ratio;
+        s_cd = sum_stripes(sc->cd_part);
     }
     __syncthreads();
+    if (threadIdx.x < kStripes) {
+        sc->nz_part[threadIdx.x] = 0;
+        sc->cd_part[threadIdx.x] = 0;
+    }
     const float ratio = s_ratio, last = s_last;
     const uint32_t t = threadIdx.x;
     uint8_t lut = sc->lut[t];
@@ -1083,7 +1117,7 @@ __global__ void __launch_bounds__(256) k_decide(View v) {
     sc->last_nz_ratio = ratio;
     sc->refresh_pending = 0;
     if (v.lr_scaling) {
-        const float cratio = (float)sc->cd_sum / (float)(v.dsW * v.dsH);
+        const float cratio = (float)s_cd / (float)(v.dsW * v.dsH);
         const uint32_t thr = v.color_thr / 2u;  // FRAMELEVEL_MIN_COLOR_DIFF_THRESHOLD
         if (sc->auto_reset) {
             if (sc->frames_since_reset > 1000u) {
@@ -1114,8 +1148,6 @@ __global__ void __launch_bounds__(256) k_decide(View v) {
         }
         if (sc->cooldown > 0u) --sc->cooldown;
     }
-    sc->nz_count = 0;
-    sc->cd_sum = 0;
     sc->frame_idx += 1u;
     sc->epoch += 1u;
 }

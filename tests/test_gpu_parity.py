@@ -49,7 +49,9 @@ def run_sequence(algo, frames, lr=None, roi=None, state_names=None, check_scalar
     for name in state_names or (SUB_STATE if algo == SUBSENSE else LOB_STATE):
         sg, so = gpu.state(name), orc.state(name)
         assert np.array_equal(sg, so), "state %s: %s" % (name, _describe_diff(sg, so))
-    if check_scalars:
+    if check_scalars and algo == LOBSTER:
+        assert gpu.scalars()["apply_count"] == orc.scalars()["apply_count"]
+    elif check_scalars:
         sg, so = gpu.scalars(), orc.scalars()
         for k in ("frame_idx", "frames_since_reset", "cooldown", "auto_reset", "lr_scaling", "use3x3", "median_k",
                   "n_roi", "lower_cap", "upper_cap", "refresh_seq"):
