@@ -140,6 +140,7 @@ def main():
     n_meas = min(steps, 200)
     ex_spec = ex_fin = wr = 0
     iters, max_iter = [], 0
+    work = np.zeros(8)
     match_ms = []
     bgs.set_timing("match")
     for i in range(n_meas):
@@ -152,6 +153,7 @@ def main():
         ex_fin += st["examined"]
         wr += st["writes"]
         iters.append(st["fixpoint_iters"])
+        work += np.array(st["fixpoint_work"], dtype=np.float64)
         max_iter = max(max_iter, st["max_fixpoint_iter"])
     bgs.set_timing("step")
     bgs.initialize(f0)
@@ -194,7 +196,8 @@ def main():
                              "p99": round(float(np.percentile(step_ms, 99)), 4),
                              "what": "GPU time of one frame step (HIP events), %d frames" % n_meas},
         "step_algorithmic_gbs": round(step_bytes / (float(np.mean(step_ms)) / 1e3) / 1e9, 1),
-        "fixpoint": {"mean_iters": round(float(np.mean(iters)), 3), "max_iter": int(max_iter)},
+        "fixpoint": {"mean_iters": round(float(np.mean(iters)), 3), "max_iter": int(max_iter),
+                     "mean_worklist": [round(float(x), 1) for x in (work / n_meas)[1:6]]},
     }
     if not args.no_cpu:
         n_cpu = 40

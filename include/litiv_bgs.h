@@ -114,6 +114,8 @@ typedef struct lvbgs_stats {
     uint32_t max_fixpoint_iter;
     uint32_t width, height, channels, n_roi;
     uint64_t examined_speculative; /* samples examined by the speculative match pass alone */
+    uint32_t fixpoint_work[8];     /* [0]: pixels finished by the cooperative match phase;
+                                      [k]: worklist size of fix-point iteration k (1..7) */
 } lvbgs_stats;
 int lvbgs_get_stats(lvbgs_ctx* ctx, lvbgs_stats* out);
 
@@ -138,9 +140,34 @@ int lvbgs_take_timing(lvbgs_ctx* ctx, double* total_ms, int* count);
 /* Fix-point schedule: full-grid iterations before the single-workgroup tail, and their grid size. */
 int lvbgs_set_fixpoint_schedule(lvbgs_ctx* ctx, int launches, int blocks);
 
+/* Match schedule: samples scanned per pixel by the sequential phase (1..255) before unfinished pixels
+ * move to the cooperative phase, and the grid of that phase. Results do not depend on it. */
+int lvbgs_set_match_schedule(lvbgs_ctx* ctx, int seq_limit, int cont_blocks);
+
 /* RNG self-check: the four draws of group 0 of counter (pixel p, frame, phase, event) for p in [0, n),
  * written to host_out[4*p .. 4*p+3] (phase 1 = apply, 2 = refresh). */
 int lvbgs_debug_rng(lvbgs_ctx* ctx, uint32_t frame, uint32_t phase, uint32_t event, uint32_t n, uint32_t* host_out);
+
+/* ---- batches (no counterpart in the reference: its changedet app runs one sequence per host
+ * thread, apps/changedet/src/main.cpp:148-152) ----
+ * A batch steps several initialized SuBSENSE contexts (any mix of sizes, 1 or 3 channels) together:
+ * one launch per kernel covers every stream. While a batch is in use its contexts must not be
+ * stepped individually. Device pointers, tightly packed; asynchronous on the batch's stream. */
+typedef struct lvbgs_batch lvbgs_batch;
+int lvbgs_batch_create(lvbgs_ctx* const* ctxs, int n, lvbgs_batch** out);
+void lvbgs_batch_destroy(lvbgs_batch* b);
+const char* lvbgs_batch_last_error(const lvbgs_batch* b);
+void* lvbgs_batch_stream(lvbgs_batch* b);
+/* use_graph: capture the step once as a hipGraph and replay it; schedule knobs as for single contexts */
+int lvbgs_batch_set_options(lvbgs_batch* b, int use_graph, int seq_limit, int cont_blocks, int fixpoint_launches,
+                            int fixpoint_blocks);
+/* one apply() of every stream: d_imgs[i] is stream i's frame, d_masks (nullable) its output mask,
+ * lrs (nullable = 0) its learning-rate override */
+int lvbgs_batch_apply_device(lvbgs_batch* b, const void* const* d_imgs, void* const* d_masks, const double* lrs);
+int lvbgs_batch_synchronize(lvbgs_batch* b);
+/* per-step GPU time of batch steps (HIP events around each step on the batch stream) */
+int lvbgs_batch_set_timing(lvbgs_batch* b, int enabled);
+int lvbgs_batch_take_timing(lvbgs_batch* b, double* total_ms, int* count);
 
 #ifdef __cplusplus
 }

@@ -52,7 +52,8 @@ class Params(C.Structure):
 class Stats(C.Structure):
     _fields_ = [("examined", C.c_uint64), ("writes", C.c_uint64), ("fixpoint_iters", C.c_uint32),
                 ("max_fixpoint_iter", C.c_uint32), ("width", C.c_uint32), ("height", C.c_uint32),
-                ("channels", C.c_uint32), ("n_roi", C.c_uint32), ("examined_speculative", C.c_uint64)]
+                ("channels", C.c_uint32), ("n_roi", C.c_uint32), ("examined_speculative", C.c_uint64),
+                ("fixpoint_work", C.c_uint32 * 8)]
 
 
 _lib = None
@@ -99,7 +100,19 @@ def lib():
         L.lvbgs_set_timing.argtypes = [vp, C.c_char_p]
         L.lvbgs_take_timing.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int)]
         L.lvbgs_set_fixpoint_schedule.argtypes = [vp, i, i]
+        L.lvbgs_set_match_schedule.argtypes = [vp, i, i]
         L.lvbgs_debug_rng.argtypes = [vp, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, vp]
+        L.lvbgs_batch_create.argtypes = [C.POINTER(vp), i, C.POINTER(vp)]
+        L.lvbgs_batch_destroy.argtypes = [vp]
+        L.lvbgs_batch_last_error.argtypes = [vp]
+        L.lvbgs_batch_last_error.restype = C.c_char_p
+        L.lvbgs_batch_stream.argtypes = [vp]
+        L.lvbgs_batch_stream.restype = vp
+        L.lvbgs_batch_set_options.argtypes = [vp, i, i, i, i, i]
+        L.lvbgs_batch_apply_device.argtypes = [vp, C.POINTER(vp), C.POINTER(vp), C.POINTER(C.c_double)]
+        L.lvbgs_batch_synchronize.argtypes = [vp]
+        L.lvbgs_batch_set_timing.argtypes = [vp, i]
+        L.lvbgs_batch_take_timing.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int)]
         _lib = L
     return _lib
 
@@ -243,7 +256,9 @@ class BackgroundSubtractor:
     def stats(self):
         s = Stats()
         self._check(lib().lvbgs_get_stats(self._h, C.byref(s)))
-        return {k: getattr(s, k) for k, _ in Stats._fields_}
+        d = {k: getattr(s, k) for k, _ in Stats._fields_}
+        d["fixpoint_work"] = list(d["fixpoint_work"])
+        return d
 
     def scalars(self):
         buf = (C.c_double * 32)()
@@ -278,6 +293,9 @@ class BackgroundSubtractor:
     def set_fixpoint_schedule(self, launches, blocks):
         self._check(lib().lvbgs_set_fixpoint_schedule(self._h, int(launches), int(blocks)))
 
+    def set_match_schedule(self, seq_limit, cont_blocks=1024):
+        self._check(lib().lvbgs_set_match_schedule(self._h, int(seq_limit), int(cont_blocks)))
+
     def debug_rng(self, frame, phase, event, n):
         out = np.empty((n, 4), np.uint32)
         self._check(lib().lvbgs_debug_rng(self._h, frame, phase, event, n, out.ctypes.data))
@@ -292,3 +310,67 @@ class BackgroundSubtractorSuBSENSE(BackgroundSubtractor):
 class BackgroundSubtractorLOBSTER(BackgroundSubtractor):
     """BackgroundSubtractorLOBSTER (BackgroundSubtractorLOBSTER.hpp:132-155); default learning rate 16."""
     ALGO = LOBSTER
+
+
+class Batch:
+    """Several initialized SuBSENSE subtractors on one device stepped together: one launch per kernel
+    covers every stream (streams may differ in size and channel count). Their results equal those
+    of stepping each subtractor alone. While the batch exists its members are stepped only through it;
+    getters and state reads of a member are valid after synchronize()."""
+
+    def __init__(self, subtractors, use_graph=True, seq_limit=255, cont_blocks=1024, fixpoint_launches=3,
+                 fixpoint_blocks=256):
+        self.members = list(subtractors)
+        hs = (C.c_void_p * len(self.members))(*[m._h.value for m in self.members])
+        h = C.c_void_p()
+        rc = lib().lvbgs_batch_create(hs, len(self.members), C.byref(h))
+        if rc != OK:
+            raise LvbgsError(rc, "lvbgs_batch_create failed: members must be distinct, initialized SuBSENSE "
+                                 "subtractors on one device")
+        self._h = h
+        self._imgs = (C.c_void_p * len(self.members))()
+        self._masks = (C.c_void_p * len(self.members))()
+        self._lrs = (C.c_double * len(self.members))()
+        self.set_options(use_graph, seq_limit, cont_blocks, fixpoint_launches, fixpoint_blocks)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().lvbgs_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def _check(self, rc):
+        if rc < 0:
+            raise LvbgsError(rc, lib().lvbgs_batch_last_error(self._h).decode())
+        return rc
+
+    def set_options(self, use_graph=True, seq_limit=255, cont_blocks=1024, fixpoint_launches=3, fixpoint_blocks=256):
+        self._check(lib().lvbgs_batch_set_options(self._h, int(bool(use_graph)), int(seq_limit), int(cont_blocks),
+                                                  int(fixpoint_launches), int(fixpoint_blocks)))
+
+    def apply_device(self, d_imgs, d_masks=None, learningRates=None):
+        """One apply() of every member: device frame pointers (and optional mask pointers) in member order."""
+        n = len(self.members)
+        if len(d_imgs) != n or (d_masks is not None and len(d_masks) != n):
+            raise LvbgsError(ERR_ARG, "one frame (and mask) per member is required")
+        for i in range(n):
+            self._imgs[i] = d_imgs[i]
+            self._masks[i] = d_masks[i] if d_masks is not None else None
+            self._lrs[i] = 0.0 if learningRates is None else float(learningRates[i])
+        self._check(lib().lvbgs_batch_apply_device(self._h, self._imgs, self._masks, self._lrs))
+
+    def synchronize(self):
+        self._check(lib().lvbgs_batch_synchronize(self._h))
+
+    def stream(self):
+        return lib().lvbgs_batch_stream(self._h)
+
+    def set_timing(self, enabled):
+        self._check(lib().lvbgs_batch_set_timing(self._h, int(bool(enabled))))
+
+    def take_timing(self):
+        ms, n = C.c_double(), C.c_int()
+        self._check(lib().lvbgs_batch_take_timing(self._h, C.byref(ms), C.byref(n)))
+        return ms.value, n.value

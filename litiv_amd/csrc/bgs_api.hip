@@ -186,6 +186,7 @@ static int allocate(lvbgs_ctx* c) {
         v.mres = ar.take<uint32_t>(npx);
         v.wl0 = ar.take<uint32_t>(npx);
         v.wl1 = ar.take<uint32_t>(npx);
+        v.cont = ar.take<uint32_t>(npx);
         v.stamp = ar.take<unsigned long long>(npx);
         v.dsLT = ar.take<float>(nds);
         v.dsST = ar.take<float>(nds);
@@ -382,7 +383,12 @@ static void step(lvbgs_ctx* c, const uint8_t* d_img, uint8_t* d_mask, double lr)
         o.ev_end = c->ev_pairs[c->ev_used].second;
         ++c->ev_used;
     }
-    frame_step(c->v, d_img, d_mask, lr, o, c->stream);
+    View v = c->v;
+    v.frame = d_img;
+    v.out_mask = d_mask;
+    v.lr = lr;
+    v.seq_limit = (uint32_t)o.seq_limit;
+    frame_step(v, o, c->stream);
     ++c->apply_count;
 }
 
@@ -540,6 +546,8 @@ extern "C" int lvbgs_get_stats(lvbgs_ctx* c, lvbgs_stats* out) {
     out->examined = sum(s.ex_part);
     out->writes = sum(s.wr_part);
     out->examined_speculative = sum(s.spec_part);
+    for (int i = 0; i < 8; ++i) out->fixpoint_work[i] = s.wl_work[i];
+    out->fixpoint_work[0] = s.cont_count;
     out->fixpoint_iters = s.iters;
     out->max_fixpoint_iter = s.max_iter_seen;
     out->width = (uint32_t)c->W;
@@ -672,6 +680,13 @@ extern "C" int lvbgs_set_fixpoint_schedule(lvbgs_ctx* c, int launches, int block
     return LVBGS_OK;
 }
 
+extern "C" int lvbgs_set_match_schedule(lvbgs_ctx* c, int seq_limit, int cont_blocks) {
+    if (!c || seq_limit < 1 || seq_limit > 255 || cont_blocks < 1 || cont_blocks > 65535) return LVBGS_ERR_ARG;
+    c->opt.seq_limit = seq_limit;
+    c->opt.cont_blocks = cont_blocks;
+    return LVBGS_OK;
+}
+
 // RNG self-check for tests: draws of group 0 for pixels [0,n) of a (frame, phase, event) counter
 extern "C" int lvbgs_debug_rng(lvbgs_ctx* c, uint32_t frame, uint32_t phase, uint32_t ev, uint32_t n, uint32_t* host_out) {
     if (!c || !host_out || !c->initialized || n == 0) return LVBGS_ERR_ARG;
@@ -683,4 +698,223 @@ extern "C" int lvbgs_debug_rng(lvbgs_ctx* c, uint32_t frame, uint32_t phase, uin
     if (!rc) rc = hip_check(c, hipStreamSynchronize(c->stream), "sync");
     hipFree(d);
     return rc;
+}
+
+// ------------------------------------------------------------------------------------------------
+// batches: several initialized SuBSENSE contexts stepped together, one launch per kernel
+// ------------------------------------------------------------------------------------------------
+struct lvbgs_batch {
+    std::vector<lvbgs_ctx*> ctx;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    View* d_views = nullptr;
+    View* h_views[2] = {nullptr, nullptr};  // pinned staging, double-buffered
+    hipEvent_t ev_copy[2] = {nullptr, nullptr};
+    bool ev_live[2] = {false, false};
+    int flip = 0;
+    uint32_t* d_prefix = nullptr;
+    const uint32_t* prefix[kNumGrids] = {};
+    uint32_t total[kNumGrids] = {};
+    StepOptions opt{};
+    bool any_cont = false;
+    bool use_graph = false;
+    hipGraphExec_t graph = nullptr;
+    int timed = 0;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pairs;
+    size_t ev_used = 0;
+};
+
+static int bset_err(lvbgs_batch* b, int code, const char* msg) {
+    if (b) b->err = msg;
+    return code;
+}
+
+static void batch_release(lvbgs_batch* b) {
+    if (b->graph) hipGraphExecDestroy(b->graph);
+    b->graph = nullptr;
+    if (b->d_views) hipFree(b->d_views);
+    if (b->d_prefix) hipFree(b->d_prefix);
+    for (int i = 0; i < 2; ++i) {
+        if (b->h_views[i]) hipHostFree(b->h_views[i]);
+        if (b->ev_copy[i]) hipEventDestroy(b->ev_copy[i]);
+    }
+    for (auto& e : b->ev_pairs) {
+        hipEventDestroy(e.first);
+        hipEventDestroy(e.second);
+    }
+    if (b->stream) hipStreamDestroy(b->stream);
+}
+
+static int batch_plan(lvbgs_batch* b) {
+    const int ns = (int)b->ctx.size();
+    std::vector<uint32_t> pre((size_t)kNumGrids * (ns + 1), 0);
+    b->any_cont = false;
+    for (int i = 0; i < ns; ++i) {
+        uint32_t g[kNumGrids];
+        kernel_grids(b->ctx[i]->v, b->opt, g);
+        for (int k = 0; k < kNumGrids; ++k) pre[(size_t)k * (ns + 1) + i + 1] = pre[(size_t)k * (ns + 1) + i] + g[k];
+        if (b->opt.seq_limit < b->ctx[i]->v.N) b->any_cont = true;
+    }
+    if (!b->d_prefix && hipMalloc(&b->d_prefix, pre.size() * sizeof(uint32_t)) != hipSuccess)
+        return bset_err(b, LVBGS_ERR_ALLOC, "batch tables");
+    if (hipMemcpy(b->d_prefix, pre.data(), pre.size() * sizeof(uint32_t), hipMemcpyHostToDevice) != hipSuccess)
+        return bset_err(b, LVBGS_ERR_HIP, "batch tables");
+    for (int k = 0; k < kNumGrids; ++k) {
+        b->prefix[k] = b->d_prefix + (size_t)k * (ns + 1);
+        b->total[k] = pre[(size_t)k * (ns + 1) + ns];
+    }
+    if (b->graph) hipGraphExecDestroy(b->graph);
+    b->graph = nullptr;
+    return LVBGS_OK;
+}
+
+extern "C" int lvbgs_batch_create(lvbgs_ctx* const* ctxs, int n, lvbgs_batch** out) {
+    if (!ctxs || n <= 0 || !out) return LVBGS_ERR_ARG;
+    *out = nullptr;
+    for (int i = 0; i < n; ++i) {
+        if (!ctxs[i] || !ctxs[i]->initialized || ctxs[i]->algo != LVBGS_SUBSENSE || ctxs[i]->device != ctxs[0]->device)
+            return LVBGS_ERR_ARG;
+        for (int j = 0; j < i; ++j)
+            if (ctxs[j] == ctxs[i]) return LVBGS_ERR_ARG;
+    }
+    lvbgs_batch* b = new lvbgs_batch();
+    b->ctx.assign(ctxs, ctxs + n);
+    b->device = ctxs[0]->device;
+    b->opt = ctxs[0]->opt;
+    hipSetDevice(b->device);
+    int rc = LVBGS_OK;
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipMalloc(&b->d_views, sizeof(View) * n) != hipSuccess ||
+        hipHostMalloc((void**)&b->h_views[0], sizeof(View) * n) != hipSuccess ||
+        hipHostMalloc((void**)&b->h_views[1], sizeof(View) * n) != hipSuccess ||
+        hipEventCreateWithFlags(&b->ev_copy[0], hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&b->ev_copy[1], hipEventDisableTiming) != hipSuccess)
+        rc = LVBGS_ERR_HIP;
+    if (!rc) rc = batch_plan(b);
+    if (rc) {
+        batch_release(b);
+        delete b;
+        return rc;
+    }
+    // member streams must be idle before the batch takes over their state
+    for (auto* c : b->ctx) hipStreamSynchronize(c->stream);
+    *out = b;
+    return LVBGS_OK;
+}
+
+extern "C" void lvbgs_batch_destroy(lvbgs_batch* b) {
+    if (!b) return;
+    hipSetDevice(b->device);
+    if (b->stream) hipStreamSynchronize(b->stream);
+    batch_release(b);
+    delete b;
+}
+
+extern "C" const char* lvbgs_batch_last_error(const lvbgs_batch* b) { return b ? b->err.c_str() : "null batch"; }
+extern "C" void* lvbgs_batch_stream(lvbgs_batch* b) { return b ? (void*)b->stream : nullptr; }
+
+extern "C" int lvbgs_batch_set_options(lvbgs_batch* b, int use_graph, int seq_limit, int cont_blocks,
+                                       int fixpoint_launches, int fixpoint_blocks) {
+    if (!b || seq_limit < 1 || seq_limit > 255 || cont_blocks < 1 || cont_blocks > 65535 || fixpoint_launches < 0 ||
+        fixpoint_launches > 64 || fixpoint_blocks < 1 || fixpoint_blocks > 65535)
+        return LVBGS_ERR_ARG;
+    hipSetDevice(b->device);
+    hipStreamSynchronize(b->stream);
+    b->use_graph = use_graph != 0;
+    b->opt.seq_limit = seq_limit;
+    b->opt.cont_blocks = cont_blocks;
+    b->opt.fixpoint_launches = fixpoint_launches;
+    b->opt.fixpoint_blocks = fixpoint_blocks;
+    return batch_plan(b);
+}
+
+extern "C" int lvbgs_batch_apply_device(lvbgs_batch* b, const void* const* d_imgs, void* const* d_masks,
+                                        const double* lrs) {
+    if (!b || !d_imgs) return LVBGS_ERR_ARG;
+    hipSetDevice(b->device);
+    const int ns = (int)b->ctx.size(), i = b->flip;
+    b->flip ^= 1;
+    // the staging buffer's previous copy (two steps ago) must have been consumed
+    if (b->ev_live[i] && hipEventSynchronize(b->ev_copy[i]) != hipSuccess) return bset_err(b, LVBGS_ERR_HIP, "sync");
+    for (int s = 0; s < ns; ++s) {
+        if (!d_imgs[s]) return bset_err(b, LVBGS_ERR_ARG, "input image must be a device pointer");
+        View v = b->ctx[s]->v;
+        v.frame = (const uint8_t*)d_imgs[s];
+        v.out_mask = d_masks ? (uint8_t*)d_masks[s] : nullptr;
+        v.lr = lrs ? lrs[s] : 0.0;
+        v.seq_limit = (uint32_t)b->opt.seq_limit;
+        b->h_views[i][s] = v;
+        ++b->ctx[s]->apply_count;
+    }
+    if (hipMemcpyAsync(b->d_views, b->h_views[i], sizeof(View) * ns, hipMemcpyHostToDevice, b->stream) != hipSuccess ||
+        hipEventRecord(b->ev_copy[i], b->stream) != hipSuccess)
+        return bset_err(b, LVBGS_ERR_HIP, "view upload");
+    b->ev_live[i] = true;
+    if (b->timed) {
+        if (b->ev_used == b->ev_pairs.size()) {
+            hipEvent_t e0, e1;
+            hipEventCreate(&e0);
+            hipEventCreate(&e1);
+            b->ev_pairs.emplace_back(e0, e1);
+        }
+        (void)hipEventRecord(b->ev_pairs[b->ev_used].first, b->stream);
+    }
+    if (b->use_graph) {
+        if (!b->graph) {
+            hipGraph_t g = nullptr;
+            if (hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
+                return bset_err(b, LVBGS_ERR_HIP, "graph capture");
+            batch_step(b->d_views, ns, b->prefix, b->total, b->opt, b->any_cont, b->stream);
+            if (hipStreamEndCapture(b->stream, &g) != hipSuccess || !g)
+                return bset_err(b, LVBGS_ERR_HIP, "graph capture");
+            const hipError_t e = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
+            hipGraphDestroy(g);
+            if (e != hipSuccess) {
+                b->graph = nullptr;
+                return bset_err(b, LVBGS_ERR_HIP, "graph instantiate");
+            }
+        }
+        if (hipGraphLaunch(b->graph, b->stream) != hipSuccess) return bset_err(b, LVBGS_ERR_HIP, "graph launch");
+    } else {
+        batch_step(b->d_views, ns, b->prefix, b->total, b->opt, b->any_cont, b->stream);
+    }
+    if (b->timed) (void)hipEventRecord(b->ev_pairs[b->ev_used++].second, b->stream);
+    return hipGetLastError() == hipSuccess ? LVBGS_OK : bset_err(b, LVBGS_ERR_HIP, "batch launch");
+}
+
+extern "C" int lvbgs_batch_synchronize(lvbgs_batch* b) {
+    if (!b) return LVBGS_ERR_ARG;
+    hipSetDevice(b->device);
+    if (hipStreamSynchronize(b->stream) != hipSuccess) return bset_err(b, LVBGS_ERR_HIP, "synchronize");
+    for (auto* c : b->ctx) {
+        uint32_t flags = 0;
+        if (hipMemcpy(&flags, &c->v.sc->err_flags, sizeof(flags), hipMemcpyDeviceToHost) != hipSuccess)
+            return bset_err(b, LVBGS_ERR_HIP, "error flags");
+        if (flags & kErrFixpointCap) return bset_err(b, LVBGS_ERR_DEVICE, "fix-point iteration cap reached");
+    }
+    return LVBGS_OK;
+}
+
+extern "C" int lvbgs_batch_set_timing(lvbgs_batch* b, int enabled) {
+    if (!b) return LVBGS_ERR_ARG;
+    b->timed = enabled ? 1 : 0;
+    b->ev_used = 0;
+    return LVBGS_OK;
+}
+
+extern "C" int lvbgs_batch_take_timing(lvbgs_batch* b, double* total_ms, int* count) {
+    if (!b || !total_ms || !count) return LVBGS_ERR_ARG;
+    if (hipStreamSynchronize(b->stream) != hipSuccess) return bset_err(b, LVBGS_ERR_HIP, "sync");
+    double tot = 0.0;
+    for (size_t i = 0; i < b->ev_used; ++i) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, b->ev_pairs[i].first, b->ev_pairs[i].second) != hipSuccess)
+            return bset_err(b, LVBGS_ERR_HIP, "event timing");
+        tot += ms;
+    }
+    *total_ms = tot;
+    *count = (int)b->ev_used;
+    b->ev_used = 0;
+    return LVBGS_OK;
 }

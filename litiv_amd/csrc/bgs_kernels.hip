@@ -172,7 +172,7 @@ LV_D void load_lut(uint8_t* lut_lds, const Scalars* sc) {
 template <int C>
 struct PxCtx {
     uint32_t cur[C], intra[C];
-    uint32_t taps[C][4];
+    uint32_t taps[C][4];                   // packed taps (byte k of word w = tap 4w+k)
     uint32_t t_sc, t_tot_col, t_tot_desc;  // SuBSENSE 3ch / LOBSTER 3ch thresholds
     uint32_t t_col, t_desc;                // 1ch thresholds
 };
@@ -213,15 +213,21 @@ LV_D void lobster_thresholds(const View& v, PxCtx<C>& px) {
 }
 
 // one sample test; returns true on a match and accumulates the SuBSENSE distances
-template <int C, int ALGO>
+template <int C, int ALGO, int RS>
 LV_D bool test_sample(const PxCtx<C>& px, const uint8_t* lut, const uint32_t col[C], const uint32_t desc[C],
                       uint32_t& tot_desc, uint32_t& tot_sum) {
     if constexpr (ALGO == kSuBSENSE && C == 3) {
+        // all colour tests first: a sample failing any of them fails whatever the descriptors say
+        uint32_t cdv[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            cdv[c] = absdiff(px.cur[c], col[c]);
+            if (cdv[c] > px.t_sc) return false;
+        }
         uint32_t td = 0, ts = 0;
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            const uint32_t cd = absdiff(px.cur[c], col[c]);
-            if (cd > px.t_sc) return false;
+            const uint32_t cd = cdv[c];
             const uint32_t inter = lbsp_threshold_packed(px.taps[c], col[c], lut[col[c]]);
             const uint32_t dd = (popc(px.intra[c] ^ desc[c]) + popc(inter ^ desc[c])) >> 1;
             uint32_t sum = (dd >> 1) * 15u + cd;
@@ -247,11 +253,16 @@ LV_D bool test_sample(const PxCtx<C>& px, const uint8_t* lut, const uint32_t col
         tot_sum = sum;
         return true;
     } else if constexpr (ALGO == kLOBSTER && C == 3) {
+        uint32_t cdv[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            cdv[c] = absdiff(px.cur[c], col[c]);
+            if (cdv[c] > px.t_sc) return false;
+        }
         uint32_t tc = 0, td = 0;
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            const uint32_t cd = absdiff(px.cur[c], col[c]);
-            if (cd > px.t_sc) return false;
+            const uint32_t cd = cdv[c];
             const uint32_t inter = lbsp_threshold_packed(px.taps[c], col[c], lut[col[c]]);
             const uint32_t dd = popc(inter ^ desc[c]);
             if (dd > px.t_desc) return false;
@@ -292,42 +303,163 @@ LV_D uint32_t candidate_writers(const View& v, int x, int y) {
 }
 
 // Ordered scan s = 0..N-1 until REQ matches (SUB:366-395, LOB:480-495). Samples overwritten by
-// applied writers (mask `applied`) are taken from the writer; the last writer in raster order wins.
-template <int C, int ALGO>
-LV_D uint32_t match_scan(const View& v, const uint8_t* frame, const uint8_t* lut, const PxCtx<C>& px, int x, int y,
-                         uint32_t applied) {
-    const uint32_t p = (uint32_t)(y * v.W + x);
+// applied writers (mask `applied`, slot bitmap `sbm`) are taken from the writer; the last writer
+// in raster order wins. Two evaluation orders give the same result as the sequential loop:
+//   scan_seq  : one lane per pixel, samples in order, kFetch loads in flight, up to `limit`;
+//   scan_coop : kGroup lanes per pixel evaluate kGroup consecutive samples at once; a ballot finds
+//               the REQ-th match in sample order, distances are reduced only up to it.
+constexpr int kFetch = 2;
+constexpr int kGroup = 8;
+constexpr uint32_t kBig = 0xFFFFu;
+
+struct ScanState {
+    uint32_t good, s, md, ms;
+};
+
+template <int C>
+LV_D ScanState scan_init() {
+    return ScanState{0u, 0u, C * 16u, C * 255u};
+}
+
+LV_D uint64_t applied_slots(const View& v, int x, int y, uint32_t applied) {
     uint64_t sbm = 0;  // slots (mod 64) touched by applied writers
     for (uint32_t m = applied; m; m &= m - 1) {
         const int i = __builtin_ctz(m);
         const uint32_t q = (uint32_t)((y + kPredDY[i]) * v.W + (x + kPredDX[i]));
         sbm |= 1ull << (rec_slot_diff(v.rec[q]) & 63u);
     }
-    const uint32_t N = (uint32_t)v.N, REQ = (uint32_t)v.REQ;
-    uint32_t good = 0, s = 0, md = C * 16u, ms = C * 255u;
-    while (good < REQ && s < N) {
-        uint32_t col[C], desc[C];
-        bool ovr = false;
-        if ((sbm >> (s & 63u)) & 1u) {
-            for (int i = 11; i >= 0 && !ovr; --i) {
-                if (!((applied >> i) & 1u)) continue;
-                const uint32_t q = (uint32_t)((y + kPredDY[i]) * v.W + (x + kPredDX[i]));
-                if (rec_slot_diff(v.rec[q]) == s) {
-                    current_value<C>(v, frame, q, col, desc);
-                    ovr = true;
-                }
+    return sbm;
+}
+
+template <int C>
+LV_D void fetch_raw(const View& v, uint32_t s, uint32_t p, uint64_t& ra, uint32_t& rb) {
+    const size_t idx = (size_t)s * v.npx + p;
+    if constexpr (C == 3) ra = ((const uint64_t*)v.sA)[idx];
+    else ra = ((const uint16_t*)v.sA)[idx];
+    rb = v.sB[idx];
+}
+
+// decodes a fetched sample, replacing it by the last applied writer's value when one wrote slot s
+template <int C>
+LV_D void sample_value(const View& v, const uint8_t* frame, int x, int y, uint32_t s, uint32_t applied, uint64_t sbm,
+                       uint64_t ra, uint32_t rb, uint32_t col[C], uint32_t desc[C]) {
+    if constexpr (C == 3) {
+        desc[0] = (uint32_t)(ra & 0xFFFFu);
+        desc[1] = (uint32_t)((ra >> 16) & 0xFFFFu);
+        desc[2] = (uint32_t)((ra >> 32) & 0xFFFFu);
+        col[0] = (uint32_t)((ra >> 48) & 0xFFu);
+        col[1] = (uint32_t)(ra >> 56);
+        col[2] = rb;
+    } else {
+        desc[0] = (uint32_t)ra;
+        col[0] = rb;
+    }
+    if ((sbm >> (s & 63u)) & 1u) {
+        for (int i = 11; i >= 0; --i) {
+            if (!((applied >> i) & 1u)) continue;
+            const uint32_t q = (uint32_t)((y + kPredDY[i]) * v.W + (x + kPredDX[i]));
+            if (rec_slot_diff(v.rec[q]) == s) {
+                current_value<C>(v, frame, q, col, desc);
+                break;
             }
         }
-        if (!ovr) load_sample<C>(v, s, p, col, desc);
-        uint32_t td = 0, ts = 0;
-        if (test_sample<C, ALGO>(px, lut, col, desc, td, ts)) {
-            md = td < md ? td : md;
-            ms = ts < ms ? ts : ms;
-            ++good;
-        }
-        ++s;
     }
-    return mres_pack(good, md, ms, s);
+}
+
+template <int C, int ALGO, int RS>
+LV_D void scan_seq(const View& v, const uint8_t* frame, const uint8_t* lut, const PxCtx<C>& px, int x, int y,
+                   uint32_t applied, uint64_t sbm, ScanState& st, uint32_t limit) {
+    const uint32_t p = (uint32_t)(y * v.W + x);
+    const uint32_t REQ = (uint32_t)v.REQ;
+    while (st.s < limit && st.good < REQ) {
+        uint64_t ra[kFetch];
+        uint32_t rb[kFetch];
+        const uint32_t s0 = st.s;
+#pragma unroll
+        for (int j = 0; j < kFetch; ++j)
+            if (s0 + j < limit) fetch_raw<C>(v, s0 + j, p, ra[j], rb[j]);
+#pragma unroll
+        for (int j = 0; j < kFetch; ++j) {
+            if (st.s >= limit || st.good >= REQ) break;
+            uint32_t col[C], desc[C];
+            sample_value<C>(v, frame, x, y, st.s, applied, sbm, ra[j], rb[j], col, desc);
+            uint32_t td = 0, ts = 0;
+            if (test_sample<C, ALGO, RS>(px, lut, col, desc, td, ts)) {
+                st.md = td < st.md ? td : st.md;
+                st.ms = ts < st.ms ? ts : st.ms;
+                ++st.good;
+            }
+            ++st.s;
+        }
+    }
+}
+
+// Called by every lane of the wavefront; lanes 8g..8g+7 share one pixel and one state (active and
+// px/x/y/applied/sbm/st identical within the group). Returns with st final.
+template <int C, int ALGO, int RS>
+LV_D void scan_coop(const View& v, const uint8_t* frame, const uint8_t* lut, const PxCtx<C>& px, int x, int y,
+                    uint32_t applied, uint64_t sbm, ScanState& st, bool active) {
+    const uint32_t N = (uint32_t)v.N, REQ = (uint32_t)v.REQ;
+    const int lane = (int)(threadIdx.x & 63u), sub = lane & (kGroup - 1), gbase = lane & ~(kGroup - 1);
+    const uint32_t p = active ? (uint32_t)(y * v.W + x) : 0u;
+    bool done = !active || st.good >= REQ || st.s >= N;
+    while (__ballot(!done) != 0ull) {
+        const uint32_t s = st.s + (uint32_t)sub;
+        bool m = false;
+        uint32_t td = kBig, ts = kBig;
+        if (!done && s < N) {
+            uint64_t ra;
+            uint32_t rb;
+            fetch_raw<C>(v, s, p, ra, rb);
+            uint32_t col[C], desc[C];
+            sample_value<C>(v, frame, x, y, s, applied, sbm, ra, rb, col, desc);
+            m = test_sample<C, ALGO, RS>(px, lut, col, desc, td, ts);
+        }
+        const uint32_t bits = (uint32_t)(__ballot(m) >> gbase) & 0xFFu;
+        uint32_t lim = kGroup;  // lanes whose sample precedes the stopping point
+        bool stop = false;
+        if (!done) {
+            const uint32_t need = REQ - st.good;
+            if ((uint32_t)__popc(bits) >= need) {
+                uint32_t c = 0, pos = 0;
+                for (uint32_t k = 0; k < (uint32_t)kGroup; ++k)
+                    if ((bits >> k) & 1u) {
+                        if (++c == need) {
+                            pos = k;
+                            break;
+                        }
+                    }
+                lim = pos + 1u;
+                stop = true;
+            }
+        }
+        uint32_t tdv = (m && (uint32_t)sub < lim) ? td : kBig, tsv = (m && (uint32_t)sub < lim) ? ts : kBig;
+#pragma unroll
+        for (int o = 1; o < kGroup; o <<= 1) {
+            const uint32_t a = (uint32_t)__shfl_xor((int)tdv, o, 64), b = (uint32_t)__shfl_xor((int)tsv, o, 64);
+            tdv = a < tdv ? a : tdv;
+            tsv = b < tsv ? b : tsv;
+        }
+        if (!done) {
+            st.md = tdv < st.md ? tdv : st.md;
+            st.ms = tsv < st.ms ? tsv : st.ms;
+            if (stop) {
+                st.good = REQ;
+                st.s += lim;
+                done = true;
+            } else {
+                st.good += (uint32_t)__popc(bits);
+                st.s = st.s + kGroup < N ? st.s + kGroup : N;
+                done = st.s >= N;
+            }
+        }
+    }
+}
+
+LV_D uint32_t pack_state(const ScanState& st) { return mres_pack(st.good, st.md, st.ms, st.s); }
+
+LV_D ScanState unpack_state(uint32_t mr) {
+    return ScanState{mr & 255u, mr >> 24, (mr >> 8) & 63u, (mr >> 14) & 1023u};
 }
 
 // worklist push with de-duplication; stamp = (epoch, iteration)
@@ -354,18 +486,22 @@ LV_D bool forward_target(const View& v, uint32_t r, int x, int y, uint32_t& t) {
 // :406-449 / LOB:499-512). Writes desc_cur, rec; counts non-zero descriptors.
 // ----------------------------------------------------------------------------------------------
 template <int C, int ALGO>
-__global__ void __launch_bounds__(TPB) k_describe(View v, const uint8_t* __restrict__ frame, double lr) {
+LV_D void describe_body(const View& v, uint32_t bid, uint32_t nblk) {
+    const uint8_t* frame = v.frame;
+    const double lr = v.lr;
     __shared__ FrameTile<C> tile;
     __shared__ uint8_t lut[256];
     const int ntx = (v.W + TW - 1) / TW, nty = (v.H + TH - 1) / TH;
-    const int tid = xcd_tile((int)blockIdx.x, ntx * nty);
+    const int tid = (int)bid;
     const int x0 = (tid % ntx) * TW, y0 = (tid / ntx) * TH;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (bid == 0 && threadIdx.x == 0) {
         // per-frame counters consumed by later kernels of this step
         for (int i = 0; i < kWlRing; ++i) v.sc->wl_count[i] = 0;
+        v.sc->cont_count = 0;
+        for (int i = 0; i < 8; ++i) v.sc->wl_work[i] = 0;
         v.sc->iters = 0;
     }
-    if (blockIdx.x == 0 && threadIdx.x < kStripes) {
+    if (bid == 0 && threadIdx.x < kStripes) {
         v.sc->ex_part[threadIdx.x] = 0;
         v.sc->wr_part[threadIdx.x] = 0;
         v.sc->spec_part[threadIdx.x] = 0;
@@ -437,120 +573,268 @@ __global__ void __launch_bounds__(TPB) k_describe(View v, const uint8_t* __restr
 }
 
 // builds the match context of ROI pixel (x,y); taps from the LDS tile or from global memory
-template <int C, int ALGO, bool FROM_TILE>
-LV_D void build_ctx(const View& v, const uint8_t* frame, const FrameTile<C>* tile, int lx, int ly, int x, int y,
-                    PxCtx<C>& px) {
+template <int C, int ALGO, int RS>
+LV_D void build_ctx(const View& v, const uint8_t* frame, const volatile uint8_t* win, int x, int y, PxCtx<C>& px) {
     const uint32_t p = (uint32_t)(y * v.W + x);
 #pragma unroll
     for (int c = 0; c < C; ++c) {
         px.cur[c] = frame[(size_t)p * C + c];
         px.intra[c] = v.desc_cur[(size_t)p * C + c];
-        if constexpr (FROM_TILE) taps_from_tile<C>(*tile, lx, ly, c, px.taps[c]);
-        else taps_from_global<C>(frame, v.W, x, y, c, px.taps[c]);
     }
+#pragma unroll
+    for (int c = 0; c < C; ++c)
+#pragma unroll
+        for (int w = 0; w < 4; ++w) {
+            uint32_t acc = 0;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const int n = 4 * w + k;
+                acc |= (uint32_t)win[(kLbspDY[n] + 2) * RS + (kLbspDX[n] + 2) * C + c] << (8 * k);
+            }
+            px.taps[c][w] = acc;
+        }
     if constexpr (ALGO == kSuBSENSE) subsense_thresholds<C>(v, v.R[p], v.unstable[p], px);
     else lobster_thresholds<C>(v, px);
+}
+
+// 5x5 patch of one pixel staged in LDS by the kGroup lanes that share it (row stride 5*C)
+template <int C>
+LV_D void stage_patch(const uint8_t* frame, int W, int x, int y, uint8_t* patch, int sub) {
+    for (int j = sub; j < 25 * C; j += kGroup) {
+        const int r = j / (5 * C), rem = j % (5 * C);
+        patch[j] = frame[((size_t)(y - 2 + r) * W + (x - 2)) * C + rem];
+    }
 }
 
 // ----------------------------------------------------------------------------------------------
 // speculative match: raster-earlier writers assumed to decide as in the previous frame
 // ----------------------------------------------------------------------------------------------
+struct PredTile;
 template <int C, int ALGO>
-LV_D uint32_t match_pixel(const View& v, const uint8_t* frame, const uint8_t* lut, const FrameTile<C>& tile, int lx,
-                          int ly, int x, int y, uint32_t p);
+LV_D bool match_pixel(const View& v, const uint8_t* frame, const uint8_t* lut, const FrameTile<C>& tile,
+                      const PredTile& pred, int lx, int ly, int x, int y, uint32_t p, uint32_t seq_limit,
+                      uint32_t& examined);
+
+// records and previous raw decisions of the raster-earlier part of a tile's 5x5 neighbourhood:
+// rows y0-2 .. y0+TH-1, columns x0-2 .. x0+TW+1
+struct PredTile {
+    static constexpr int LW = TW + 4, LH = TH + 2;
+    uint32_t rec[LH][LW];
+    uint8_t guess[LH][LW];
+};
+
+LV_D void load_pred_tile(PredTile& t, const View& v, int x0, int y0) {
+    for (int i = threadIdx.x; i < PredTile::LW * PredTile::LH; i += TPB) {
+        const int ly = i / PredTile::LW, lx = i % PredTile::LW, gx = x0 + lx - 2, gy = y0 + ly - 2;
+        const bool in = gx >= 0 && gx < v.W && gy >= 0 && gy < v.H;
+        const size_t q = (size_t)gy * v.W + gx;
+        t.rec[ly][lx] = in ? v.rec[q] : kRecNone;
+        t.guess[ly][lx] = in ? (v.last_raw[q] ? 1 : 0) : 1;
+    }
+}
+
+// candidate writers of tile-local pixel (lx,ly) whose guessed decision is background
+LV_D uint32_t applied_writers_tile(const PredTile& t, int lx, int ly) {
+    uint32_t m = 0;
+#pragma unroll
+    for (int i = 0; i < 12; ++i) {
+        const int ty = ly + 2 + kPredDY[i], tx = lx + 2 + kPredDX[i];
+        const uint32_t r = t.rec[ty][tx];
+        const uint32_t code = (uint32_t)((2 - kPredDY[i]) * 5 + (2 - kPredDX[i]));
+        if (rec_tgt(r) == code && fwd_gate(r) && t.guess[ty][tx] == 0) m |= 1u << i;
+    }
+    return m;
+}
 
 template <int C, int ALGO>
-__global__ void __launch_bounds__(TPB) k_match(View v, const uint8_t* __restrict__ frame) {
+LV_D void match_body(const View& v, uint32_t bid, uint32_t nblk) {
+    const uint8_t* frame = v.frame;
+    const uint32_t seq_limit = v.seq_limit;
     __shared__ FrameTile<C> tile;
+    __shared__ PredTile pred;
     __shared__ uint8_t lut[256];
     const int ntx = (v.W + TW - 1) / TW, nty = (v.H + TH - 1) / TH;
-    const int tid = xcd_tile((int)blockIdx.x, ntx * nty);
+    const int tid = (int)bid;
     const int x0 = (tid % ntx) * TW, y0 = (tid / ntx) * TH;
     load_frame_tile<C>(tile, frame, v.W, v.H, x0, y0);
+    load_pred_tile(pred, v, x0, y0);
     load_lut(lut, v.sc);
     __syncthreads();
     const int lx = threadIdx.x % TW, ly = threadIdx.x / TW, x = x0 + lx, y = y0 + ly;
     const uint32_t p = (uint32_t)(y * v.W + x);
     uint32_t examined = 0;
-    if (x < v.W && y < v.H && v.roi[p]) examined = match_pixel<C, ALGO>(v, frame, lut, tile, lx, ly, x, y, p);
+    bool cont = false;
+    if (x < v.W && y < v.H && v.roi[p])
+        cont = !match_pixel<C, ALGO>(v, frame, lut, tile, pred, lx, ly, x, y, p, seq_limit, examined);
+    // unfinished pixels go to the cooperative phase (one atomic per wavefront)
+    const unsigned long long bal = __ballot(cont);
+    if (bal) {
+        const int lane = (int)(threadIdx.x & 63u), leader = __ffsll((long long)bal) - 1;
+        uint32_t base = 0;
+        if (lane == leader) base = atomicAdd(&v.sc->cont_count, (uint32_t)__popcll(bal));
+        base = (uint32_t)__shfl((int)base, leader, 64);
+        if (cont) v.cont[base + (uint32_t)__popcll(bal & ((1ull << lane) - 1ull))] = p;
+    }
     block_add<TPB>(examined, v.sc->spec_part);
 }
 
+// Phase A: sequential scan of at most v_seq_limit samples. Returns true when the pixel is final;
+// otherwise its partial state is left in mres for the cooperative phase.
 template <int C, int ALGO>
-LV_D uint32_t match_pixel(const View& v, const uint8_t* frame, const uint8_t* lut, const FrameTile<C>& tile, int lx,
-                          int ly, int x, int y, uint32_t p) {
+LV_D bool match_pixel(const View& v, const uint8_t* frame, const uint8_t* lut, const FrameTile<C>& tile,
+                      const PredTile& pred, int lx, int ly, int x, int y, uint32_t p, uint32_t seq_limit,
+                      uint32_t& examined) {
     PxCtx<C> px;
-    build_ctx<C, ALGO, true>(v, frame, &tile, lx, ly, x, y, px);
-    const uint32_t cand = candidate_writers(v, x, y);
-    uint32_t applied = 0;
-    for (uint32_t m = cand; m; m &= m - 1) {
-        const int i = __builtin_ctz(m);
-        const uint32_t q = (uint32_t)((y + kPredDY[i]) * v.W + (x + kPredDX[i]));
-        if (v.last_raw[q] == 0) applied |= 1u << i;  // guess: background as in the last frame
-    }
-    const uint32_t mr = match_scan<C, ALGO>(v, frame, lut, px, x, y, applied);
-    const uint8_t fg = (mr & 255u) < (uint32_t)v.REQ ? 1 : 0;
-    v.mres[p] = mr;
+    build_ctx<C, ALGO, FrameTile<C>::LW * C>(v, frame, &tile.px[(ly * FrameTile<C>::LW + lx) * C], x, y, px);
+    // guess: raster-earlier writers decide as in the previous frame
+    const uint32_t applied = applied_writers_tile(pred, lx, ly);
+    const uint64_t sbm = applied_slots(v, x, y, applied);
+    ScanState st = scan_init<C>();
+    const uint32_t N = (uint32_t)v.N;
+    scan_seq<C, ALGO, FrameTile<C>::LW * C>(v, frame, lut, px, x, y, applied, sbm, st, seq_limit < N ? seq_limit : N);
+    examined = st.s;
+    v.mres[p] = pack_state(st);
+    if (st.good < (uint32_t)v.REQ && st.s < N) return false;
+    const uint8_t fg = st.good < (uint32_t)v.REQ ? 1 : 0;
     v.dec[p] = fg;
     const uint8_t guess = v.last_raw[p] ? 1 : 0;
     uint32_t t;
     if (fg != guess && forward_target(v, v.rec[p], x, y, t)) wl_push(v, t, 1u, v.sc->epoch, v.wl1);
-    return mr >> 24;
+    return true;
 }
 
-// re-match of one worklisted pixel with the current decisions of its writers
-template <int C, int ALGO>
-LV_D void rematch(const View& v, const uint8_t* frame, const uint8_t* lut, uint32_t p, uint32_t k, uint64_t epoch,
-                  uint32_t* wl_next) {
-    const int x = (int)(p % (uint32_t)v.W), y = (int)(p / (uint32_t)v.W);
-    PxCtx<C> px;
-    build_ctx<C, ALGO, false>(v, frame, nullptr, 0, 0, x, y, px);
+// candidate writers of p whose decision is background: guessed from the previous raw mask, or the
+// current decisions during the fix point
+template <bool CURRENT>
+LV_D uint32_t applied_writers_global(const View& v, int x, int y) {
     const uint32_t cand = candidate_writers(v, x, y);
     uint32_t applied = 0;
     for (uint32_t m = cand; m; m &= m - 1) {
         const int i = __builtin_ctz(m);
         const uint32_t q = (uint32_t)((y + kPredDY[i]) * v.W + (x + kPredDX[i]));
-        if (__hip_atomic_load(&v.dec[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) applied |= 1u << i;
+        const uint8_t d = CURRENT ? __hip_atomic_load(&v.dec[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                                  : (v.last_raw[q] ? 1 : 0);
+        if (d == 0) applied |= 1u << i;
     }
-    const uint32_t mr = match_scan<C, ALGO>(v, frame, lut, px, x, y, applied);
-    const uint8_t fg = (mr & 255u) < (uint32_t)v.REQ ? 1 : 0;
-    v.mres[p] = mr;
-    const uint8_t old = v.dec[p];
-    if (fg != old) {
-        __hip_atomic_store(&v.dec[p], fg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        uint32_t t;
-        if (forward_target(v, v.rec[p], x, y, t)) wl_push(v, t, k + 1u, epoch, wl_next);
+    return applied;
+}
+
+// Phase B: pixels left unfinished by phase A (mostly foreground), kGroup lanes per pixel
+template <int C, int ALGO>
+LV_D void match_cont_body(const View& v, uint32_t bid, uint32_t nblk) {
+    const uint8_t* frame = v.frame;
+    __shared__ uint8_t lut[256];
+    __shared__ uint8_t patches[TPB / kGroup][25 * C];
+    load_lut(lut, v.sc);
+    __syncthreads();
+    uint8_t* patch = patches[threadIdx.x / kGroup];
+    const uint32_t n = v.sc->cont_count;
+    const uint32_t ngroups = nblk * (TPB / kGroup);
+    const uint32_t sub = threadIdx.x & (kGroup - 1);
+    const uint32_t g0 = (bid * TPB + threadIdx.x) / kGroup;
+    const uint32_t rounds = (n + ngroups - 1) / ngroups;  // uniform trip count: every lane takes part
+    const uint64_t epoch = v.sc->epoch;
+    uint64_t examined = 0;
+    for (uint32_t r = 0; r < rounds; ++r) {
+        const uint32_t i = g0 + r * ngroups;
+        const bool active = i < n;
+        const uint32_t p = active ? v.cont[i] : 0u;
+        const int x = (int)(p % (uint32_t)v.W), y = (int)(p / (uint32_t)v.W);
+        PxCtx<C> px;
+        uint32_t applied = 0;
+        uint64_t sbm = 0;
+        ScanState st = scan_init<C>();
+        __syncthreads();  // previous round's readers of the patch are done
+        if (active) {
+            stage_patch<C>(frame, v.W, x, y, patch, (int)sub);
+            build_ctx<C, ALGO, 5 * C>(v, frame, patch, x, y, px);
+            applied = applied_writers_global<false>(v, x, y);
+            sbm = applied_slots(v, x, y, applied);
+            st = unpack_state(v.mres[p]);
+        }
+        __syncthreads();
+        const uint32_t s_before = st.s;
+        scan_coop<C, ALGO, 5 * C>(v, frame, lut, px, x, y, applied, sbm, st, active);
+        if (active && sub == 0) {
+            examined += st.s - s_before;
+            v.mres[p] = pack_state(st);
+            const uint8_t fg = st.good < (uint32_t)v.REQ ? 1 : 0;
+            v.dec[p] = fg;
+            const uint8_t guess = v.last_raw[p] ? 1 : 0;
+            uint32_t t;
+            if (fg != guess && forward_target(v, v.rec[p], x, y, t)) wl_push(v, t, 1u, epoch, v.wl1);
+        }
+    }
+    block_add<TPB>(examined, v.sc->spec_part);
+}
+
+// fix-point re-match of worklisted pixels (all lanes call; kGroup lanes per item)
+template <int C, int ALGO>
+LV_D void rematch_round(const View& v, const uint8_t* frame, const uint8_t* lut, uint8_t* patch, const uint32_t* wl,
+                        uint32_t n, uint32_t i, uint32_t k, uint64_t epoch, uint32_t* wl_next) {
+    const bool active = i < n;
+    const uint32_t p = active ? wl[i] : 0u;
+    const int x = (int)(p % (uint32_t)v.W), y = (int)(p / (uint32_t)v.W);
+    PxCtx<C> px;
+    uint32_t applied = 0;
+    uint64_t sbm = 0;
+    __syncthreads();  // previous round's readers of the patch are done
+    if (active) {
+        stage_patch<C>(frame, v.W, x, y, patch, (int)(threadIdx.x & (kGroup - 1)));
+        build_ctx<C, ALGO, 5 * C>(v, frame, patch, x, y, px);
+        applied = applied_writers_global<true>(v, x, y);
+        sbm = applied_slots(v, x, y, applied);
+    }
+    __syncthreads();
+    ScanState st = scan_init<C>();
+    scan_coop<C, ALGO, 5 * C>(v, frame, lut, px, x, y, applied, sbm, st, active);
+    if (active && (threadIdx.x & (kGroup - 1)) == 0) {
+        v.mres[p] = pack_state(st);
+        const uint8_t fg = st.good < (uint32_t)v.REQ ? 1 : 0;
+        const uint8_t old = v.dec[p];
+        if (fg != old) {
+            __hip_atomic_store(&v.dec[p], fg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            uint32_t t;
+            if (forward_target(v, v.rec[p], x, y, t)) wl_push(v, t, k + 1u, epoch, wl_next);
+        }
     }
 }
 
 // fix-point iteration k (k >= 1): worklist k -> worklist k+1
 template <int C, int ALGO>
-__global__ void __launch_bounds__(TPB) k_fixpoint(View v, const uint8_t* __restrict__ frame, uint32_t k) {
+LV_D void fixpoint_body(const View& v, uint32_t bid, uint32_t nblk, uint32_t k) {
+    const uint8_t* frame = v.frame;
     __shared__ uint8_t lut[256];
+    __shared__ uint8_t patches[TPB / kGroup][25 * C];
     // slot k+2 was last read by iteration k-2 (finished); it must be empty before iteration k+1
     // pushes into it, whether or not this iteration has work.
-    if (blockIdx.x == 0 && threadIdx.x == 0)
+    if (bid == 0 && threadIdx.x == 0)
         __hip_atomic_store(&v.sc->wl_count[(k + 2) % kWlRing], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     const uint32_t n = __hip_atomic_load(&v.sc->wl_count[k % kWlRing], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (n == 0) return;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (bid == 0 && threadIdx.x == 0) {
         atomicAdd(&v.sc->iters, 1u);
         atomicMax(&v.sc->max_iter_seen, k);
+        if (k < 8) v.sc->wl_work[k] = n;
     }
     load_lut(lut, v.sc);
     __syncthreads();
     const uint32_t* wl = (k & 1u) ? v.wl1 : v.wl0;
     uint32_t* wl_next = (k & 1u) ? v.wl0 : v.wl1;
     const uint64_t epoch = v.sc->epoch;
-    for (uint32_t i = blockIdx.x * TPB + threadIdx.x; i < n; i += gridDim.x * TPB)
-        rematch<C, ALGO>(v, frame, lut, wl[i], k, epoch, wl_next);
+    const uint32_t ngroups = nblk * (TPB / kGroup);
+    const uint32_t g0 = (bid * TPB + threadIdx.x) / kGroup;
+    const uint32_t rounds = (n + ngroups - 1) / ngroups;
+    for (uint32_t r = 0; r < rounds; ++r)
+        rematch_round<C, ALGO>(v, frame, lut, patches[threadIdx.x / kGroup], wl, n, g0 + r * ngroups, k, epoch, wl_next);
 }
 
 // remaining fix-point iterations in one workgroup (normally no work is left)
 template <int C, int ALGO>
-__global__ void __launch_bounds__(1024) k_fixpoint_tail(View v, const uint8_t* __restrict__ frame, uint32_t k0,
-                                                        uint32_t cap) {
+LV_D void fixpoint_tail_body(const View& v, uint32_t bid, uint32_t nblk, uint32_t k0, uint32_t cap) {
+    const uint8_t* frame = v.frame;
     __shared__ uint8_t lut[256];
+    __shared__ uint8_t patches[256 / kGroup][25 * C];
     __shared__ uint32_t s_n;
     load_lut(lut, v.sc);
     const uint64_t epoch = v.sc->epoch;
@@ -567,6 +851,7 @@ __global__ void __launch_bounds__(1024) k_fixpoint_tail(View v, const uint8_t* _
                                        __HIP_MEMORY_SCOPE_AGENT);
                     v.sc->iters += 1u;
                     v.sc->max_iter_seen = k > v.sc->max_iter_seen ? k : v.sc->max_iter_seen;
+                    if (k < 8) v.sc->wl_work[k] = s_n;
                 }
             }
         }
@@ -575,7 +860,10 @@ __global__ void __launch_bounds__(1024) k_fixpoint_tail(View v, const uint8_t* _
         if (n == 0) break;
         const uint32_t* wl = (k & 1u) ? v.wl1 : v.wl0;
         uint32_t* wl_next = (k & 1u) ? v.wl0 : v.wl1;
-        for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) rematch<C, ALGO>(v, frame, lut, wl[i], k, epoch, wl_next);
+        const uint32_t ngroups = blockDim.x / kGroup, g0 = threadIdx.x / kGroup;
+        const uint32_t rounds = (n + ngroups - 1) / ngroups;
+        for (uint32_t r = 0; r < rounds; ++r)
+            rematch_round<C, ALGO>(v, frame, lut, patches[threadIdx.x / kGroup], wl, n, g0 + r * ngroups, k, epoch, wl_next);
         __threadfence();
     }
 }
@@ -585,8 +873,9 @@ __global__ void __launch_bounds__(1024) k_fixpoint_tail(View v, const uint8_t* _
 // (SUB:536-539). Whole image: pixels outside the ROI get raw = 0.
 // ----------------------------------------------------------------------------------------------
 template <int C>
-__global__ void __launch_bounds__(TPB) k_feedback(View v, const uint8_t* __restrict__ frame) {
-    const uint32_t p = blockIdx.x * TPB + threadIdx.x;
+LV_D void feedback_body(const View& v, uint32_t bid, uint32_t nblk) {
+    const uint8_t* frame = v.frame;
+    const uint32_t p = bid * TPB + threadIdx.x;
     const bool valid = p < v.npx;
     uint32_t examined = 0;
     uint8_t raw = 0;
@@ -682,8 +971,8 @@ __global__ void __launch_bounds__(TPB) k_feedback(View v, const uint8_t* __restr
 }
 
 // LOBSTER: raw mask (LOB:496-497, :554-555) and previous-raw bookkeeping for the next speculation
-__global__ void __launch_bounds__(TPB) k_lobster_raw(View v) {
-    const uint32_t p = blockIdx.x * TPB + threadIdx.x;
+LV_D void lobster_raw_body(const View& v, uint32_t bid, uint32_t nblk) {
+    const uint32_t p = bid * TPB + threadIdx.x;
     uint32_t examined = 0;
     if (p < v.npx) {
         uint8_t raw = 0;
@@ -702,59 +991,73 @@ __global__ void __launch_bounds__(TPB) k_lobster_raw(View v) {
 // the raster order of the writers; the last write of each slot wins (SUB:406-449, LOB:498-512).
 // ----------------------------------------------------------------------------------------------
 template <int C, int ALGO>
-__global__ void __launch_bounds__(TPB) k_update(View v, const uint8_t* __restrict__ frame) {
-    const uint32_t iw = (uint32_t)(v.W - 4), ih = (uint32_t)(v.H - 4);
-    const uint32_t i = blockIdx.x * TPB + threadIdx.x;
+LV_D void update_body(const View& v, uint32_t bid, uint32_t nblk) {
+    const uint8_t* frame = v.frame;
+    constexpr int LW = TW + 4, LH = TH + 4;
+    __shared__ uint32_t s_rec[LH][LW];  // records of the tile's 5x5 neighbourhood, tgt = kRecNone when
+    __shared__ uint8_t s_bg[LH][LW];    // the writer is outside the image; s_bg: writer decided background
+    const int ntx = (v.W + TW - 1) / TW, nty = (v.H + TH - 1) / TH;
+    const int tid = (int)bid;
+    const int x0 = (tid % ntx) * TW, y0 = (tid / ntx) * TH;
+    for (int i = threadIdx.x; i < LW * LH; i += TPB) {
+        const int ly = i / LW, lx = i % LW, gx = x0 + lx - 2, gy = y0 + ly - 2;
+        const bool in = gx >= 0 && gx < v.W && gy >= 0 && gy < v.H;
+        const size_t q = (size_t)gy * v.W + gx;
+        const uint32_t r = in ? v.rec[q] : kRecNone;
+        s_rec[ly][lx] = r;
+        s_bg[ly][lx] = (rec_tgt(r) != kRecNone && v.dec[q] == 0) ? 1 : 0;
+    }
+    __syncthreads();
+    const int lx = threadIdx.x % TW, ly = threadIdx.x / TW, x = x0 + lx, y = y0 + ly;
     uint32_t nw = 0;
-    if (i < iw * ih) {
-        const int x = 2 + (int)(i % iw), y = 2 + (int)(i / iw);
+    if (x >= 2 && y >= 2 && x < v.W - 2 && y < v.H - 2) {
         const uint32_t p = (uint32_t)(y * v.W + x);
         // ghost test of writers at or after p reads p's updated maps (SUB:438-441)
         bool ghost_new = false;
         if constexpr (ALGO == kSuBSENSE) ghost_new = v.RawST[p] > 0.995f && v.MLD[p] < 0.010f;
-        uint8_t slot[26];
-        uint8_t src[26];
-        int n = 0;
+        // Writers are visited in reverse raster order, so the first write seen for a slot is the
+        // last one the sequential loop performs; later-seen writes to a written slot are skipped.
+        uint64_t done0 = 0, done1 = 0, done2 = 0, done3 = 0;  // slots already final (N <= 255)
 #pragma unroll 1
-        for (int j = 0; j < 25; ++j) {
+        for (int j = 24; j >= 0; --j) {
             const int dy = j / 5 - 2, dx = j % 5 - 2;
-            const uint32_t q = (uint32_t)((y + dy) * v.W + (x + dx));
-            const uint32_t r = v.rec[q];
+            const uint32_t r = s_rec[ly + 2 + dy][lx + 2 + dx];
             const uint32_t tg = rec_tgt(r);
             if (tg == kRecNone) continue;
-            const bool bg = v.dec[q] == 0;
-            if (j == 12) {  // the pixel itself: self update (or foreground replacement) first
-                if (bg ? rec_self_bg(r) : (ALGO == kSuBSENSE && rec_self_fg(r))) {
-                    slot[n] = (uint8_t)rec_slot_self(r);
-                    src[n++] = 12;
-                }
-            }
-            if (!bg || tg != (uint32_t)((2 - dy) * 5 + (2 - dx))) continue;
-            const bool gate = j < 12 ? fwd_gate(r) : (rec_gate_a(r) || (rec_gate_b(r) && ghost_new));
-            if (gate) {
-                slot[n] = (uint8_t)rec_slot_diff(r);
-                src[n++] = (uint8_t)j;
+            const bool bg = s_bg[ly + 2 + dy][lx + 2 + dx] != 0;
+            // up to two writes by this writer, in sequential order: self update (j == 12), then diffusion
+            uint32_t wslot[2];
+            int nwr = 0;
+            if (j == 12 && (bg ? rec_self_bg(r) : (ALGO == kSuBSENSE && rec_self_fg(r)))) wslot[nwr++] = rec_slot_self(r);
+            if (bg && tg == (uint32_t)((2 - dy) * 5 + (2 - dx)) &&
+                (j < 12 ? fwd_gate(r) : (rec_gate_a(r) || (rec_gate_b(r) && ghost_new))))
+                wslot[nwr++] = rec_slot_diff(r);
+            for (int k = nwr - 1; k >= 0; --k) {
+                const uint32_t sl = wslot[k];
+                const uint64_t bit = 1ull << (sl & 63u);
+                const uint32_t word = sl >> 6;
+                const uint64_t dw = word == 0 ? done0 : word == 1 ? done1 : word == 2 ? done2 : done3;
+                ++nw;
+                if (dw & bit) continue;
+                if (word == 0) done0 |= bit;
+                else if (word == 1) done1 |= bit;
+                else if (word == 2) done2 |= bit;
+                else done3 |= bit;
+                const uint32_t q = (uint32_t)((y + dy) * v.W + (x + dx));
+                uint32_t col[C], desc[C];
+                current_value<C>(v, frame, q, col, desc);
+                store_sample<C>(v, sl, p, col, desc);
             }
         }
-        for (int a = n - 1; a >= 0; --a) {
-            bool later = false;
-            for (int b = a + 1; b < n; ++b) later |= slot[b] == slot[a];
-            if (later) continue;
-            const int j = src[a];
-            const uint32_t q = (uint32_t)((y + j / 5 - 2) * v.W + (x + j % 5 - 2));
-            uint32_t col[C], desc[C];
-            current_value<C>(v, frame, q, col, desc);
-            store_sample<C>(v, slot[a], p, col, desc);
-        }
-        nw = (uint32_t)n;
     }
     block_add<TPB>(nw, v.sc->wr_part);
 }
 
 // LOBSTER: last colour frame := input (LOB:580)
 template <int C>
-__global__ void __launch_bounds__(TPB) k_copy_last_color(View v, const uint8_t* __restrict__ frame) {
-    const uint32_t i = blockIdx.x * TPB + threadIdx.x;
+LV_D void copy_last_color_body(const View& v, uint32_t bid, uint32_t nblk) {
+    const uint8_t* frame = v.frame;
+    const uint32_t i = bid * TPB + threadIdx.x;
     if (i < v.npx * (uint32_t)C) v.last_color[i] = frame[i];
 }
 
@@ -764,12 +1067,12 @@ __global__ void __launch_bounds__(TPB) k_copy_last_color(View v, const uint8_t* 
 constexpr int PW = 64, PH = 16;  // post-processing tile; 256 threads x 4 pixels
 
 // morphologyEx CLOSE with a 3x3 rectangle, neutral border: erode3(dilate3(raw))
-__global__ void __launch_bounds__(TPB) k_close(View v) {
+LV_D void close_body(const View& v, uint32_t bid, uint32_t nblk) {
     constexpr int LW = PW + 4, LH = PH + 4;
     __shared__ uint8_t s_raw[LH][LW];
     __shared__ uint8_t s_dil[LH - 2][LW - 2];
     const int ntx = (v.W + PW - 1) / PW, nty = (v.H + PH - 1) / PH;
-    const int tid = xcd_tile((int)blockIdx.x, ntx * nty);
+    const int tid = (int)bid;
     const int x0 = (tid % ntx) * PW, y0 = (tid / ntx) * PH;
     for (int i = threadIdx.x; i < LW * LH; i += TPB) {
         const int ly = i / LW, lx = i % LW, gx = x0 + lx - 2, gy = y0 + ly - 2;
@@ -801,10 +1104,10 @@ __global__ void __launch_bounds__(TPB) k_close(View v) {
 // --- hole fill: union-find labelling of the 0-pixels of PreFlood (4-connectivity) ---
 constexpr int CT = 32;  // CCL tile (32x32)
 
-__global__ void __launch_bounds__(TPB) k_ccl_local(View v) {
+LV_D void ccl_local_body(const View& v, uint32_t bid, uint32_t nblk) {
     __shared__ uint32_t lab[CT * CT];
     const int ntx = (v.W + CT - 1) / CT;
-    const int x0 = ((int)blockIdx.x % ntx) * CT, y0 = ((int)blockIdx.x / ntx) * CT;
+    const int x0 = ((int)bid % ntx) * CT, y0 = ((int)bid / ntx) * CT;
     for (int i = threadIdx.x; i < CT * CT; i += TPB) {
         const int gx = x0 + i % CT, gy = y0 + i / CT;
         const bool bgpx = gx < v.W && gy < v.H && v.pre[(size_t)gy * v.W + gx] == 0;
@@ -886,10 +1189,10 @@ LV_D void g_union(uint32_t* lab, uint32_t a, uint32_t b) {
 }
 
 // unions across CCL tile borders (right column and bottom row of every tile)
-__global__ void __launch_bounds__(TPB) k_ccl_border(View v) {
+LV_D void ccl_border_body(const View& v, uint32_t bid, uint32_t nblk) {
     const int ntx = (v.W + CT - 1) / CT, nty = (v.H + CT - 1) / CT;
     const uint32_t per_tile = 2 * CT;
-    const uint32_t i = blockIdx.x * TPB + threadIdx.x;
+    const uint32_t i = bid * TPB + threadIdx.x;
     if (i >= (uint32_t)(ntx * nty) * per_tile) return;
     const int t = (int)(i / per_tile), k = (int)(i % per_tile);
     const int x0 = (t % ntx) * CT, y0 = (t / ntx) * CT;
@@ -911,8 +1214,8 @@ __global__ void __launch_bounds__(TPB) k_ccl_border(View v) {
 
 // holes = 0-pixels of PreFlood not connected to pixel (0,0) (floodFill + bitwise_not, SUB:541-543);
 // mh = raw | holes
-__global__ void __launch_bounds__(TPB) k_holes(View v) {
-    const uint32_t p = blockIdx.x * TPB + threadIdx.x;
+LV_D void holes_body(const View& v, uint32_t bid, uint32_t nblk) {
+    const uint32_t p = bid * TPB + threadIdx.x;
     if (p >= v.npx) return;
     uint8_t hole = 0;
     if (v.pre[p] == 0) {
@@ -926,7 +1229,7 @@ __global__ void __launch_bounds__(TPB) k_holes(View v) {
 // mask = raw | holes | erode7(PreFlood); last_fg = medianBlur(mask, k) (SUB:544-547; LOB:578 uses
 // the raw mask directly). Binary median with zero padding = majority count over the k x k window.
 template <bool SUBSENSE>
-__global__ void __launch_bounds__(TPB) k_median(View v) {
+LV_D void median_body(const View& v, uint32_t bid, uint32_t nblk) {
     constexpr int HM = 6, HE = 3;                  // median halo (k <= 13), erosion halo
     constexpr int MW = PW + 2 * HM, MH = PH + 2 * HM;  // mask region
     constexpr int EW = MW + 2 * HE, EH = MH + 2 * HE;  // PreFlood region
@@ -935,7 +1238,7 @@ __global__ void __launch_bounds__(TPB) k_median(View v) {
     __shared__ uint8_t s_mask[MH][MW];
     __shared__ uint8_t s_col[PH][MW];
     const int ntx = (v.W + PW - 1) / PW, nty = (v.H + PH - 1) / PH;
-    const int tid = xcd_tile((int)blockIdx.x, ntx * nty);
+    const int tid = (int)bid;
     const int x0 = (tid % ntx) * PW, y0 = (tid / ntx) * PH;
     const int k = v.median_k, r = k / 2, need = (k * k + 1) / 2;
     if constexpr (SUBSENSE) {
@@ -985,12 +1288,13 @@ __global__ void __launch_bounds__(TPB) k_median(View v) {
 
 // dilate7(last_fg), blink masks (SUB:548-551), final-segmentation averages (SUB:553-554) and the
 // output mask. addWeighted(.., CV_32F) is evaluated as the double-precision scalar form.
-__global__ void __launch_bounds__(TPB) k_dilate_final(View v, uint8_t* __restrict__ out_mask) {
+LV_D void dilate_final_body(const View& v, uint32_t bid, uint32_t nblk) {
+    uint8_t* out_mask = v.out_mask;
     constexpr int H3 = 3, LW = PW + 2 * H3, LH = PH + 2 * H3;
     __shared__ uint8_t s_fg[LH][LW];
     __shared__ uint8_t s_h[LH][PW];
     const int ntx = (v.W + PW - 1) / PW, nty = (v.H + PH - 1) / PH;
-    const int tid = xcd_tile((int)blockIdx.x, ntx * nty);
+    const int tid = (int)bid;
     const int x0 = (tid % ntx) * PW, y0 = (tid / ntx) * PH;
     for (int i = threadIdx.x; i < LW * LH; i += TPB) {
         const int ly = i / LW, lx = i % LW, gx = x0 + lx - H3, gy = y0 + ly - H3;
@@ -1026,8 +1330,10 @@ __global__ void __launch_bounds__(TPB) k_dilate_final(View v, uint8_t* __restric
     }
 }
 
-__global__ void __launch_bounds__(TPB) k_copy_mask(View v, uint8_t* __restrict__ out_mask) {
-    const uint32_t p = blockIdx.x * TPB + threadIdx.x;
+LV_D void copy_mask_body(const View& v, uint32_t bid, uint32_t nblk) {
+    uint8_t* out_mask = v.out_mask;
+    if (!out_mask) return;
+    const uint32_t p = bid * TPB + threadIdx.x;
     if (p < v.npx) out_mask[p] = v.last_fg[p];
 }
 
@@ -1041,8 +1347,9 @@ LV_D uint8_t sat_u8(float f) {
 
 // resize(INTER_AREA, W/8 x H/8) + accumulateWeighted LT/ST + per-cell |ST-LT| (SUB:568-583)
 template <int C>
-__global__ void __launch_bounds__(TPB) k_downsample(View v, const uint8_t* __restrict__ frame) {
-    const uint32_t cell = blockIdx.x * TPB + threadIdx.x;
+LV_D void downsample_body(const View& v, uint32_t bid, uint32_t nblk) {
+    const uint8_t* frame = v.frame;
+    const uint32_t cell = bid * TPB + threadIdx.x;
     const uint32_t ncell = (uint32_t)(v.dsW * v.dsH);
     uint32_t contrib = 0;
     if (cell < ncell) {
@@ -1087,7 +1394,7 @@ __global__ void __launch_bounds__(TPB) k_downsample(View v, const uint8_t* __res
 }
 
 // single-workgroup decision: LUT adaptation, model reset, learning-rate caps, counters
-__global__ void __launch_bounds__(256) k_decide(View v) {
+LV_D void decide_body(const View& v, uint32_t bid, uint32_t nblk) {
     Scalars* sc = v.sc;
     __shared__ float s_ratio, s_last;
     __shared__ unsigned long long s_cd;
@@ -1153,7 +1460,8 @@ __global__ void __launch_bounds__(256) k_decide(View v) {
 }
 
 // LOBSTER end of step: frame counter and worklist epoch
-__global__ void k_lobster_end(View v) {
+LV_D void lobster_end_body(const View& v, uint32_t bid, uint32_t nblk) {
+    if (threadIdx.x != 0) return;
     v.sc->frame_idx += 1u;
     v.sc->epoch += 1u;
 }
@@ -1161,8 +1469,7 @@ __global__ void k_lobster_end(View v) {
 // refreshModel (SUB:80-105; LOB:410-440). pending = 1: auto-reset request from k_decide (also
 // T := 1 everywhere, SUB:592); otherwise an explicit call with the given parameters.
 template <int C, int ALGO>
-__global__ void __launch_bounds__(TPB) k_refresh(View v, int use_pending, float frac, int force_fg, uint32_t frame,
-                                               uint32_t ev) {
+LV_D void refresh_body(const View& v, uint32_t bid, uint32_t nblk, int use_pending, float frac, int force_fg, uint32_t frame, uint32_t ev) {
     if (use_pending) {
         if (!v.sc->refresh_pending) return;
         frac = v.sc->refresh_frac;
@@ -1170,7 +1477,7 @@ __global__ void __launch_bounds__(TPB) k_refresh(View v, int use_pending, float 
         frame = v.sc->refresh_frame;
         ev = v.sc->refresh_ev;
     }
-    const uint32_t p = blockIdx.x * TPB + threadIdx.x;
+    const uint32_t p = bid * TPB + threadIdx.x;
     if (p >= v.npx) return;
     if (use_pending && ALGO == kSuBSENSE) v.T[p] = 1.0f;
     if (!v.roi[p]) return;
@@ -1292,59 +1599,196 @@ __global__ void k_rng_probe(View v, uint32_t frame, uint32_t phase, uint32_t ev,
 }
 
 // ==============================================================================================
+// kernel entry points. Single stream: the View by value, block id remapped per XCD. Batched: every
+// stream of the batch in one launch; a workgroup finds its stream in a per-kernel prefix table of
+// block counts (after the XCD remap, so each stream's tiles stay together) and dispatches on the
+// stream's channel count.
+// ==============================================================================================
+struct BatchArgs {
+    const View* views;
+    const uint32_t* prefix;  // ns+1 block offsets
+    int ns;
+};
+
+template <class F>
+LV_D void for_batch_block(const BatchArgs& b, F&& f) {
+    const uint32_t lb = (uint32_t)xcd_tile((int)blockIdx.x, (int)gridDim.x);
+    int lo = 0, hi = b.ns;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (b.prefix[mid] <= lb) lo = mid;
+        else hi = mid;
+    }
+    // private copy: the fields become uniform registers that stores to the model cannot alias
+    const View v = b.views[lo];
+    f(v, lb - b.prefix[lo], b.prefix[lo + 1] - b.prefix[lo]);
+}
+
+#define LV_SINGLE_BID ((uint32_t)xcd_tile((int)blockIdx.x, (int)gridDim.x)), gridDim.x
+
+template <int C, int ALGO>
+__global__ void __launch_bounds__(TPB) k_describe(View v) { describe_body<C, ALGO>(v, LV_SINGLE_BID); }
+template <int C, int ALGO>
+__global__ void __launch_bounds__(TPB) k_match(View v) { match_body<C, ALGO>(v, LV_SINGLE_BID); }
+template <int C, int ALGO>
+__global__ void __launch_bounds__(TPB) k_match_cont(View v) { match_cont_body<C, ALGO>(v, LV_SINGLE_BID); }
+template <int C, int ALGO>
+__global__ void __launch_bounds__(TPB) k_fixpoint(View v, uint32_t k) { fixpoint_body<C, ALGO>(v, LV_SINGLE_BID, k); }
+template <int C, int ALGO>
+__global__ void __launch_bounds__(256) k_fixpoint_tail(View v, uint32_t k0, uint32_t cap) {
+    fixpoint_tail_body<C, ALGO>(v, 0u, 1u, k0, cap);
+}
+template <int C>
+__global__ void __launch_bounds__(TPB) k_feedback(View v) { feedback_body<C>(v, LV_SINGLE_BID); }
+__global__ void __launch_bounds__(TPB) k_lobster_raw(View v) { lobster_raw_body(v, LV_SINGLE_BID); }
+template <int C, int ALGO>
+__global__ void __launch_bounds__(TPB) k_update(View v) { update_body<C, ALGO>(v, LV_SINGLE_BID); }
+template <int C>
+__global__ void __launch_bounds__(TPB) k_copy_last_color(View v) { copy_last_color_body<C>(v, LV_SINGLE_BID); }
+__global__ void __launch_bounds__(TPB) k_close(View v) { close_body(v, LV_SINGLE_BID); }
+__global__ void __launch_bounds__(TPB) k_ccl_local(View v) { ccl_local_body(v, LV_SINGLE_BID); }
+__global__ void __launch_bounds__(TPB) k_ccl_border(View v) { ccl_border_body(v, LV_SINGLE_BID); }
+__global__ void __launch_bounds__(TPB) k_holes(View v) { holes_body(v, LV_SINGLE_BID); }
+template <bool SUB>
+__global__ void __launch_bounds__(TPB) k_median(View v) { median_body<SUB>(v, LV_SINGLE_BID); }
+__global__ void __launch_bounds__(TPB) k_dilate_final(View v) { dilate_final_body(v, LV_SINGLE_BID); }
+__global__ void __launch_bounds__(TPB) k_copy_mask(View v) { copy_mask_body(v, LV_SINGLE_BID); }
+template <int C>
+__global__ void __launch_bounds__(TPB) k_downsample(View v) { downsample_body<C>(v, LV_SINGLE_BID); }
+__global__ void __launch_bounds__(256) k_decide(View v) { decide_body(v, 0u, 1u); }
+__global__ void __launch_bounds__(64) k_lobster_end(View v) { lobster_end_body(v, 0u, 1u); }
+template <int C, int ALGO>
+__global__ void __launch_bounds__(TPB) k_refresh(View v, int use_pending, float frac, int force_fg, uint32_t frame,
+                                               uint32_t ev) {
+    refresh_body<C, ALGO>(v, LV_SINGLE_BID, use_pending, frac, force_fg, frame, ev);
+}
+
+// batched entry points (SuBSENSE; streams of 1 and 3 channels may be mixed)
+#define LV_BATCH_C(body, ...)                                                                \
+    for_batch_block(b, [&](const View& v, uint32_t bid, uint32_t nblk) {                     \
+        if (v.C == 3) body<3 __VA_OPT__(, ) __VA_ARGS__>(v, bid, nblk);                       \
+        else body<1 __VA_OPT__(, ) __VA_ARGS__>(v, bid, nblk);                                \
+    })
+__global__ void __launch_bounds__(TPB) kb_describe(BatchArgs b) { LV_BATCH_C(describe_body, kSuBSENSE); }
+__global__ void __launch_bounds__(TPB) kb_match(BatchArgs b) { LV_BATCH_C(match_body, kSuBSENSE); }
+__global__ void __launch_bounds__(TPB) kb_match_cont(BatchArgs b) { LV_BATCH_C(match_cont_body, kSuBSENSE); }
+__global__ void __launch_bounds__(TPB) kb_fixpoint(BatchArgs b, uint32_t k) {
+    for_batch_block(b, [&](const View& v, uint32_t bid, uint32_t nblk) {
+        if (v.C == 3) fixpoint_body<3, kSuBSENSE>(v, bid, nblk, k);
+        else fixpoint_body<1, kSuBSENSE>(v, bid, nblk, k);
+    });
+}
+__global__ void __launch_bounds__(256) kb_fixpoint_tail(BatchArgs b, uint32_t k0, uint32_t cap) {
+    for_batch_block(b, [&](const View& v, uint32_t bid, uint32_t nblk) {
+        if (v.C == 3) fixpoint_tail_body<3, kSuBSENSE>(v, bid, nblk, k0, cap);
+        else fixpoint_tail_body<1, kSuBSENSE>(v, bid, nblk, k0, cap);
+    });
+}
+__global__ void __launch_bounds__(TPB) kb_feedback(BatchArgs b) { LV_BATCH_C(feedback_body); }
+__global__ void __launch_bounds__(TPB) kb_update(BatchArgs b) { LV_BATCH_C(update_body, kSuBSENSE); }
+__global__ void __launch_bounds__(TPB) kb_close(BatchArgs b) { for_batch_block(b, close_body); }
+__global__ void __launch_bounds__(TPB) kb_ccl_local(BatchArgs b) { for_batch_block(b, ccl_local_body); }
+__global__ void __launch_bounds__(TPB) kb_ccl_border(BatchArgs b) { for_batch_block(b, ccl_border_body); }
+__global__ void __launch_bounds__(TPB) kb_holes(BatchArgs b) { for_batch_block(b, holes_body); }
+__global__ void __launch_bounds__(TPB) kb_median(BatchArgs b) { for_batch_block(b, median_body<true>); }
+__global__ void __launch_bounds__(TPB) kb_dilate_final(BatchArgs b) { for_batch_block(b, dilate_final_body); }
+__global__ void __launch_bounds__(TPB) kb_downsample(BatchArgs b) { LV_BATCH_C(downsample_body); }
+__global__ void __launch_bounds__(256) kb_decide(BatchArgs b) { for_batch_block(b, decide_body); }
+__global__ void __launch_bounds__(TPB) kb_refresh(BatchArgs b) {
+    for_batch_block(b, [&](const View& v, uint32_t bid, uint32_t nblk) {
+        if (v.C == 3) refresh_body<3, kSuBSENSE>(v, bid, nblk, 1, 0.f, 0, 0u, 0u);
+        else refresh_body<1, kSuBSENSE>(v, bid, nblk, 1, 0.f, 0, 0u, 0u);
+    });
+}
+
+// ==============================================================================================
 // host-side launchers
 // ==============================================================================================
 static inline unsigned cdiv(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }
 
+void kernel_grids(const View& v, const StepOptions& opt, uint32_t g[kNumGrids]) {
+    const unsigned ct = cdiv(v.W, CT) * cdiv(v.H, CT);
+    g[kGridTile] = cdiv(v.W, TW) * cdiv(v.H, TH);
+    g[kGridPost] = cdiv(v.W, PW) * cdiv(v.H, PH);
+    g[kGridCcl] = ct;
+    g[kGridCclBorder] = cdiv((size_t)ct * 2 * CT, TPB);
+    g[kGridPix] = cdiv(v.npx, TPB);
+    g[kGridCell] = v.lr_scaling ? cdiv((size_t)v.dsW * v.dsH, TPB) : 0u;
+    g[kGridOne] = 1u;
+    g[kGridFix] = (uint32_t)opt.fixpoint_blocks;
+    g[kGridCont] = (uint32_t)opt.cont_blocks;
+}
+
 template <int C, int ALGO>
-static void frame_step_t(const View& v, const uint8_t* frame, uint8_t* out_mask, double lr, const StepOptions& opt,
-                         hipStream_t st) {
-    const unsigned tiles = cdiv(v.W, TW) * cdiv(v.H, TH);
-    const unsigned ptiles = cdiv(v.W, PW) * cdiv(v.H, PH);
-    const unsigned pix_blocks = cdiv(v.npx, TPB);
-    const unsigned inner_blocks = cdiv((size_t)(v.W - 4) * (v.H - 4), TPB);
+static void frame_step_t(const View& v, const StepOptions& opt, hipStream_t st) {
+    uint32_t g[kNumGrids];
+    kernel_grids(v, opt, g);
     if (opt.timed_stage == 2) (void)hipEventRecord(opt.ev_begin, st);
-    hipLaunchKernelGGL((k_describe<C, ALGO>), dim3(tiles), dim3(TPB), 0, st, v, frame, lr);
+    hipLaunchKernelGGL((k_describe<C, ALGO>), dim3(g[kGridTile]), dim3(TPB), 0, st, v);
     if (opt.timed_stage == 1) (void)hipEventRecord(opt.ev_begin, st);
-    hipLaunchKernelGGL((k_match<C, ALGO>), dim3(tiles), dim3(TPB), 0, st, v, frame);
+    hipLaunchKernelGGL((k_match<C, ALGO>), dim3(g[kGridTile]), dim3(TPB), 0, st, v);
+    if (opt.seq_limit < v.N) hipLaunchKernelGGL((k_match_cont<C, ALGO>), dim3(g[kGridCont]), dim3(TPB), 0, st, v);
     if (opt.timed_stage == 1) (void)hipEventRecord(opt.ev_end, st);
     for (uint32_t k = 1; k <= (uint32_t)opt.fixpoint_launches; ++k)
-        hipLaunchKernelGGL((k_fixpoint<C, ALGO>), dim3(opt.fixpoint_blocks), dim3(TPB), 0, st, v, frame, k);
-    hipLaunchKernelGGL((k_fixpoint_tail<C, ALGO>), dim3(1), dim3(1024), 0, st, v, frame,
-                       (uint32_t)opt.fixpoint_launches + 1u, (uint32_t)opt.tail_cap);
+        hipLaunchKernelGGL((k_fixpoint<C, ALGO>), dim3(g[kGridFix]), dim3(TPB), 0, st, v, k);
+    hipLaunchKernelGGL((k_fixpoint_tail<C, ALGO>), dim3(1), dim3(256), 0, st, v, (uint32_t)opt.fixpoint_launches + 1u,
+                       (uint32_t)opt.tail_cap);
     if constexpr (ALGO == kSuBSENSE) {
-        hipLaunchKernelGGL((k_feedback<C>), dim3(pix_blocks), dim3(TPB), 0, st, v, frame);
-        hipLaunchKernelGGL((k_update<C, ALGO>), dim3(inner_blocks), dim3(TPB), 0, st, v, frame);
-        hipLaunchKernelGGL(k_close, dim3(ptiles), dim3(TPB), 0, st, v);
-        const unsigned ct = cdiv(v.W, CT) * cdiv(v.H, CT);
-        hipLaunchKernelGGL(k_ccl_local, dim3(ct), dim3(TPB), 0, st, v);
-        hipLaunchKernelGGL(k_ccl_border, dim3(cdiv((size_t)ct * 2 * CT, TPB)), dim3(TPB), 0, st, v);
-        hipLaunchKernelGGL(k_holes, dim3(pix_blocks), dim3(TPB), 0, st, v);
-        hipLaunchKernelGGL((k_median<true>), dim3(ptiles), dim3(TPB), 0, st, v);
-        hipLaunchKernelGGL(k_dilate_final, dim3(ptiles), dim3(TPB), 0, st, v, out_mask);
-        if (v.lr_scaling) hipLaunchKernelGGL((k_downsample<C>), dim3(cdiv((size_t)v.dsW * v.dsH, TPB)), dim3(TPB), 0, st, v, frame);
+        hipLaunchKernelGGL((k_feedback<C>), dim3(g[kGridPix]), dim3(TPB), 0, st, v);
+        hipLaunchKernelGGL((k_update<C, ALGO>), dim3(g[kGridTile]), dim3(TPB), 0, st, v);
+        hipLaunchKernelGGL(k_close, dim3(g[kGridPost]), dim3(TPB), 0, st, v);
+        hipLaunchKernelGGL(k_ccl_local, dim3(g[kGridCcl]), dim3(TPB), 0, st, v);
+        hipLaunchKernelGGL(k_ccl_border, dim3(g[kGridCclBorder]), dim3(TPB), 0, st, v);
+        hipLaunchKernelGGL(k_holes, dim3(g[kGridPix]), dim3(TPB), 0, st, v);
+        hipLaunchKernelGGL((k_median<true>), dim3(g[kGridPost]), dim3(TPB), 0, st, v);
+        hipLaunchKernelGGL(k_dilate_final, dim3(g[kGridPost]), dim3(TPB), 0, st, v);
+        if (g[kGridCell]) hipLaunchKernelGGL((k_downsample<C>), dim3(g[kGridCell]), dim3(TPB), 0, st, v);
         hipLaunchKernelGGL(k_decide, dim3(1), dim3(256), 0, st, v);
-        hipLaunchKernelGGL((k_refresh<C, ALGO>), dim3(pix_blocks), dim3(TPB), 0, st, v, 1, 0.f, 0, 0u, 0u);
+        hipLaunchKernelGGL((k_refresh<C, ALGO>), dim3(g[kGridPix]), dim3(TPB), 0, st, v, 1, 0.f, 0, 0u, 0u);
     } else {
-        hipLaunchKernelGGL(k_lobster_raw, dim3(pix_blocks), dim3(TPB), 0, st, v);
-        hipLaunchKernelGGL((k_update<C, ALGO>), dim3(inner_blocks), dim3(TPB), 0, st, v, frame);
-        hipLaunchKernelGGL((k_median<false>), dim3(ptiles), dim3(TPB), 0, st, v);
-        if (out_mask) hipLaunchKernelGGL(k_copy_mask, dim3(pix_blocks), dim3(TPB), 0, st, v, out_mask);
-        hipLaunchKernelGGL((k_copy_last_color<C>), dim3(cdiv((size_t)v.npx * C, TPB)), dim3(TPB), 0, st, v, frame);
-        hipLaunchKernelGGL(k_lobster_end, dim3(1), dim3(1), 0, st, v);
+        hipLaunchKernelGGL(k_lobster_raw, dim3(g[kGridPix]), dim3(TPB), 0, st, v);
+        hipLaunchKernelGGL((k_update<C, ALGO>), dim3(g[kGridTile]), dim3(TPB), 0, st, v);
+        hipLaunchKernelGGL((k_median<false>), dim3(g[kGridPost]), dim3(TPB), 0, st, v);
+        if (v.out_mask) hipLaunchKernelGGL(k_copy_mask, dim3(g[kGridPix]), dim3(TPB), 0, st, v);
+        hipLaunchKernelGGL((k_copy_last_color<C>), dim3(cdiv((size_t)v.npx * C, TPB)), dim3(TPB), 0, st, v);
+        hipLaunchKernelGGL(k_lobster_end, dim3(1), dim3(64), 0, st, v);
     }
     if (opt.timed_stage == 2) (void)hipEventRecord(opt.ev_end, st);
 }
 
-void frame_step(const View& v, const uint8_t* frame, uint8_t* out_mask, double lr, const StepOptions& opt,
-                hipStream_t st) {
+void frame_step(const View& v, const StepOptions& opt, hipStream_t st) {
     if (v.algo == kSuBSENSE) {
-        if (v.C == 3) frame_step_t<3, kSuBSENSE>(v, frame, out_mask, lr, opt, st);
-        else frame_step_t<1, kSuBSENSE>(v, frame, out_mask, lr, opt, st);
+        if (v.C == 3) frame_step_t<3, kSuBSENSE>(v, opt, st);
+        else frame_step_t<1, kSuBSENSE>(v, opt, st);
     } else {
-        if (v.C == 3) frame_step_t<3, kLOBSTER>(v, frame, out_mask, lr, opt, st);
-        else frame_step_t<1, kLOBSTER>(v, frame, out_mask, lr, opt, st);
+        if (v.C == 3) frame_step_t<3, kLOBSTER>(v, opt, st);
+        else frame_step_t<1, kLOBSTER>(v, opt, st);
     }
+}
+
+// one SuBSENSE step of every stream of a batch; d_views holds the streams' Views (with their
+// per-frame inputs), prefix[k] the (ns+1) block offsets of grid kind k, total[k] their sums
+void batch_step(const View* d_views, int ns, const uint32_t* const prefix[kNumGrids], const uint32_t total[kNumGrids],
+                const StepOptions& opt, bool any_cont, hipStream_t st) {
+    auto args = [&](int kind) { return BatchArgs{d_views, prefix[kind], ns}; };
+    hipLaunchKernelGGL(kb_describe, dim3(total[kGridTile]), dim3(TPB), 0, st, args(kGridTile));
+    hipLaunchKernelGGL(kb_match, dim3(total[kGridTile]), dim3(TPB), 0, st, args(kGridTile));
+    if (any_cont) hipLaunchKernelGGL(kb_match_cont, dim3(total[kGridCont]), dim3(TPB), 0, st, args(kGridCont));
+    for (uint32_t k = 1; k <= (uint32_t)opt.fixpoint_launches; ++k)
+        hipLaunchKernelGGL(kb_fixpoint, dim3(total[kGridFix]), dim3(TPB), 0, st, args(kGridFix), k);
+    hipLaunchKernelGGL(kb_fixpoint_tail, dim3(total[kGridOne]), dim3(256), 0, st, args(kGridOne),
+                       (uint32_t)opt.fixpoint_launches + 1u, (uint32_t)opt.tail_cap);
+    hipLaunchKernelGGL(kb_feedback, dim3(total[kGridPix]), dim3(TPB), 0, st, args(kGridPix));
+    hipLaunchKernelGGL(kb_update, dim3(total[kGridTile]), dim3(TPB), 0, st, args(kGridTile));
+    hipLaunchKernelGGL(kb_close, dim3(total[kGridPost]), dim3(TPB), 0, st, args(kGridPost));
+    hipLaunchKernelGGL(kb_ccl_local, dim3(total[kGridCcl]), dim3(TPB), 0, st, args(kGridCcl));
+    hipLaunchKernelGGL(kb_ccl_border, dim3(total[kGridCclBorder]), dim3(TPB), 0, st, args(kGridCclBorder));
+    hipLaunchKernelGGL(kb_holes, dim3(total[kGridPix]), dim3(TPB), 0, st, args(kGridPix));
+    hipLaunchKernelGGL(kb_median, dim3(total[kGridPost]), dim3(TPB), 0, st, args(kGridPost));
+    hipLaunchKernelGGL(kb_dilate_final, dim3(total[kGridPost]), dim3(TPB), 0, st, args(kGridPost));
+    if (total[kGridCell]) hipLaunchKernelGGL(kb_downsample, dim3(total[kGridCell]), dim3(TPB), 0, st, args(kGridCell));
+    hipLaunchKernelGGL(kb_decide, dim3(total[kGridOne]), dim3(256), 0, st, args(kGridOne));
+    hipLaunchKernelGGL(kb_refresh, dim3(total[kGridPix]), dim3(TPB), 0, st, args(kGridPix));
 }
 
 void launch_refresh(const View& v, float frac, int force_fg, uint32_t frame, uint32_t ev, hipStream_t st) {

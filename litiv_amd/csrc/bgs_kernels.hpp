@@ -7,16 +7,36 @@
 namespace lvbgs {
 
 struct StepOptions {
-    int fixpoint_launches = 6;   // fix-point iterations launched as full-grid kernels
+    int fixpoint_launches = 3;   // fix-point iterations launched as full-grid kernels
     int fixpoint_blocks = 256;   // grid of each fix-point kernel (grid-stride over the worklist)
     int tail_cap = 1 << 16;      // iteration cap of the single-workgroup tail (error flag if reached)
+    int seq_limit = 255;         // samples scanned per pixel by the sequential phase of the match
+    int cont_blocks = 1024;      // grid of the cooperative phase (grid-stride over unfinished pixels)
     int timed_stage = 0;         // 0: none, 1: the match kernel, 2: the whole step
     hipEvent_t ev_begin = nullptr, ev_end = nullptr;
 };
 
-// one apply() step; frame is device memory (W*H*C), out_mask optional device W*H
-void frame_step(const View& v, const uint8_t* frame, uint8_t* out_mask, double lr, const StepOptions& opt,
-                hipStream_t st);
+// grid kinds of the frame-step kernels (block counts per stream)
+enum GridKind : int {
+    kGridTile = 0,   // 64x4 pixel tiles (describe, match, update)
+    kGridPost,       // 64x16 tiles (close, median, dilate)
+    kGridCcl,        // 32x32 tiles (local labelling)
+    kGridCclBorder,  // tile-border unions
+    kGridPix,        // 256 pixels per block
+    kGridCell,       // 256 frame-level cells per block (0 when learning-rate scaling is off)
+    kGridOne,        // one block per stream
+    kGridFix,        // fix-point iteration grid
+    kGridCont,       // cooperative match grid
+    kNumGrids
+};
+void kernel_grids(const View& v, const StepOptions& opt, uint32_t g[kNumGrids]);
+
+// one apply() step; v.frame is device memory (W*H*C), v.out_mask optional device W*H
+void frame_step(const View& v, const StepOptions& opt, hipStream_t st);
+
+// one SuBSENSE step of every stream of a batch (see bgs_kernels.hip)
+void batch_step(const View* d_views, int ns, const uint32_t* const prefix[kNumGrids], const uint32_t total[kNumGrids],
+                const StepOptions& opt, bool any_cont, hipStream_t st);
 void launch_refresh(const View& v, float frac, int force_fg, uint32_t frame, uint32_t ev, hipStream_t st);
 void launch_init_desc(const View& v, const uint8_t* img, hipStream_t st);
 void launch_background(const View& v, uint8_t* out_img, uint16_t* out_desc, hipStream_t st);

@@ -20,6 +20,11 @@ struct View {
     float rel_lbsp;          // m_fRelLBSPThreshold
     uint32_t seed_lo, seed_hi, stream_id;
     int use3x3, median_k, lr_scaling, dsW, dsH;
+    // per-step inputs
+    const uint8_t* frame;    // current input frame, device W*H*C
+    uint8_t* out_mask;       // optional device W*H output mask
+    double lr;               // learning-rate override of apply()
+    uint32_t seq_limit;      // samples of the sequential match phase
     // roi and model
     const uint8_t* roi;      // W*H, 0 = outside
     uint8_t* last_color;     // W*H*C
@@ -39,6 +44,7 @@ struct View {
     uint8_t* dec;            // W*H raw decisions (1 = foreground)
     uint32_t* wl0;           // worklists (n_roi entries each)
     uint32_t* wl1;
+    uint32_t* cont;          // pixels left to the cooperative phase of the match (n_roi entries)
     unsigned long long* stamp;  // W*H worklist de-duplication stamps
     float *dsLT, *dsST;      // (W/8)*(H/8)*C
     Scalars* sc;

@@ -34,8 +34,11 @@ def _describe_diff(a, b):
     return "%d values differ" % int((a != b).sum())
 
 
-def run_sequence(algo, frames, lr=None, roi=None, state_names=None, check_scalars=True, **params):
+def run_sequence(algo, frames, lr=None, roi=None, state_names=None, check_scalars=True, schedule=None, **params):
     gpu, orc = _pair(algo, **params)
+    if schedule is not None:
+        gpu.set_match_schedule(*schedule[:2])
+        gpu.set_fixpoint_schedule(*schedule[2:])
     gpu.initialize(frames[0], roi)
     orc.initialize(frames[0], roi)
     for t, f in enumerate(frames):
@@ -93,6 +96,13 @@ def test_subsense_gray_qvga():
 def test_subsense_rgb_vga_5x5_spread():
     # 640x480 > 2*QVGA: 5x5 diffusion and median 13
     run_sequence(SUBSENSE, seq(n=15, width=640, height=480, n_objects=4, obj_min=30, obj_max=80))
+
+
+@pytest.mark.parametrize("schedule", [(1, 7, 0, 1), (255, 1024, 6, 256), (3, 1, 1, 1)])
+def test_subsense_schedule_independent(schedule):
+    # the match / fix-point schedule (sequential depth, cooperative grid, full-grid fix-point launches)
+    # changes the order of evaluation, never the result
+    run_sequence(SUBSENSE, seq(n=25, width=160, height=120), schedule=schedule)
 
 
 @pytest.mark.parametrize("lr", [float("inf"), 3.0, 1.0])
