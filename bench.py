@@ -1,22 +1,27 @@
 """Benchmark of the gfx950 SuBSENSE / LOBSTER frame step on synthetic sequences resident in HBM.
 
 Contract: `python bench.py --gpus N --steps K --warmup W` prints ONE JSON line. A step is one
-apply() of one frame of the workload's sequence; the W warm-up frames are applied first (untimed),
-then exactly K frames are timed between two device synchronizations. Frames are generated on the
-GPU before the timed region (identical to the host generator's frames).
+apply() of one frame of every stream of the workload; the W warm-up steps are applied first
+(untimed), then exactly K steps are timed between two device synchronizations. Frames are generated
+on the GPU before the timed region (identical to the host generator's frames).
 
-Default workload (one GPU): SuBSENSE plain drop-in on 1920x1080 RGB synthetic video
-(12 moving rectangles of 40-200 px, noise sigma 4, illumination ramp, flickering patch).
-Extra fields: roofline of the dominant kernel (speculative match; algorithmic bytes from the
-device's exact examined-sample counter), the CPU baseline (sequential oracle, one host thread, on a
-bounded prefix of the same sequence), per-frame GPU step latency p50/p99, fix-point statistics.
+Default workload: 64 SuBSENSE streams (16x 320x240 RGB, 16x 640x480 RGB, 16x 640x512 grey,
+16x 1920x1080 RGB; independent generator seeds; 600 frames each) stepped as one batch (one launch
+per kernel for all streams, replayed from a hipGraph). The same 64 streams are also run serially
+through the plain one-stream drop-in (apply + synchronize per stream) over the same frames, and
+the line carries both rates and their ratio.
+
+Extra fields: roofline of the dominant kernel (the speculative match; algorithmic bytes from the
+device's exact examined-sample counters), the CPU baseline (the sequential oracle on a bounded
+sample of the same streams, one stream per host thread), per-step GPU latency p50/p99, fix-point
+statistics. Plain single-stream workloads (1080p, QVGA, 4K, LOBSTER thermal) are selectable.
 """
 import argparse
 import json
-import math
 import os
 import platform
 import sys
+import threading
 import time
 
 import numpy as np
@@ -25,13 +30,15 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+METRIC = "SuBSENSE Mpix/s: 64-stream batched vs 1-stream plain; p99 frame latency"
 
 WORKLOADS = {
-    # name: (algo, synth preset, overrides, frames in the reference configuration)
-    "subsense_1080p_plain": ("subsense", 1, {}, 1000),
-    "subsense_qvga_plain": ("subsense", 0, {}, 300),
-    "subsense_4k_plain": ("subsense", 2, {}, 500),
-    "lobster_640x512_gray_plain": ("lobster", 3, {}, 1000),
+    # name: (kind, algo, synth preset, streams, frames in the reference configuration)
+    "subsense_batch64": ("batch", "subsense", 4, 64, 600),
+    "subsense_1080p_plain": ("plain", "subsense", 1, 1, 1000),
+    "subsense_qvga_plain": ("plain", "subsense", 0, 1, 300),
+    "subsense_4k_plain": ("plain", "subsense", 2, 1, 500),
+    "lobster_640x512_gray_plain": ("plain", "lobster", 3, 1, 1000),
 }
 
 
@@ -39,63 +46,245 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def match_bytes_per_launch(C, n_roi, examined):
-    """Algorithmic bytes of one speculative-match launch: per ROI pixel the input frame (C), intra
-    descriptors (2C), R (4), previous unstable flag (1), update record (4), previous raw decision (1),
-    match result (4) and decision (1) written, plus 3C bytes per examined sample (colour C + descriptor 2C)."""
+def match_bytes(C, n_roi, examined):
+    """Algorithmic bytes of one speculative-match pass over a stream: per ROI pixel the input frame (C),
+    intra descriptors (2C), R (4), previous unstable flag (1), update record (4), previous raw decision
+    (1), match result (4) and decision (1) written, plus 3C bytes per examined sample (colour C +
+    descriptor 2C)."""
     return n_roi * (3 * C + 15) + 3 * C * examined
 
 
-def step_bytes_per_frame(C, n_roi, examined, writes):
+def step_bytes(C, n_roi, examined, writes):
     """Algorithmic bytes of a whole frame step (per-pixel figure 114 + 9(k+w) for RGB, 100 + 3(k+w) grey)."""
-    base = 114 if C == 3 else 100
-    return n_roi * base + 3 * C * (examined + writes)
+    return n_roi * (114 if C == 3 else 100) + 3 * C * (examined + writes)
 
 
-def cpu_baseline(algo, frames_host, lr, budget_s):
-    """Times the sequential oracle (one host thread) on a prefix of the same sequence."""
+# ------------------------------------------------------------------------------------------------
+# CPU baseline: the sequential oracle, one stream per host thread (ctypes releases the GIL)
+# ------------------------------------------------------------------------------------------------
+def _oracle_rate(algo, classes, lr, budget_s, threads):
+    """classes: list of (weight_px_per_step, [frames of one stream per thread]). Each class runs with
+    `threads` threads (each on its own stream) for about budget_s/len(classes) seconds; returns the
+    mix rate (pixels of one workload step / CPU time of one step) and per-class rates."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from oracle_lib import LOBSTER, SUBSENSE, Oracle
-    o = Oracle(SUBSENSE if algo == "subsense" else LOBSTER)
-    o.initialize(frames_host[0])
-    h, w = frames_host[0].shape[:2]
-    t_apply, n = 0.0, 0
-    for f in frames_host:
+    per_class = budget_s / len(classes)
+    rates, descr = [], []
+    for weight, streams in classes:
+        streams = streams[:threads]
+        orcs = []
+        for sid, frames in streams:
+            o = Oracle(SUBSENSE if algo == "subsense" else LOBSTER, stream_id=sid)
+            o.initialize(frames[0])
+            orcs.append(o)
+        done = [0] * len(streams)
+        px = [f[0].shape[0] * f[0].shape[1] for _, f in streams]
+        stop = time.perf_counter() + per_class
+
+        def work(i):
+            frames = streams[i][1]
+            while time.perf_counter() < stop and done[i] < len(frames) - 1:
+                orcs[i].apply(frames[1 + done[i]], lr)
+                done[i] += 1
+
         t0 = time.perf_counter()
-        o.apply(f, lr)
-        t_apply += time.perf_counter() - t0
-        n += 1
-        if t_apply > budget_s:
-            break
-    return {"value": round(h * w * n / t_apply / 1e6, 4), "unit": "Mpix/s", "cores": 1, "kind": "port",
-            "sample": "%d frames %dx%d of the same sequence (frames 0..%d), sequential CPU oracle, 1 thread, %s" % (
-                n, w, h, n - 1, platform.processor() or platform.machine())}
+        ths = [threading.Thread(target=work, args=(i,)) for i in range(len(streams))]
+        for t in ths:
+            t.start()
+        for t in ths:
+            t.join()
+        wall = time.perf_counter() - t0
+        rate = sum(px[i] * done[i] for i in range(len(streams))) / wall
+        rates.append((weight, rate))
+        descr.append("%dx%d: %d frames over %d thread(s)" % (streams[0][1][0].shape[1], streams[0][1][0].shape[0],
+                                                               sum(done), len(streams)))
+    total_px = sum(w for w, _ in rates)
+    step_s = sum(w / r for w, r in rates)
+    return total_px / step_s / 1e6, descr
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=None)
-    ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default="subsense_1080p_plain", choices=sorted(WORKLOADS))
-    ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of oracle time for the CPU baseline")
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--profile-steps", type=int, default=0, help="stop after this many timed steps (profiling runs)")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print("bench.py: this build measures one GPU; multi-GPU runs (--gpus %d) are not part of it. "
-              "Streams shard across GPUs with no exchange; see DESIGN.md." % args.gpus, file=sys.stderr)
-        sys.exit(2)
+def cpu_baseline(algo, classes, lr, budget_s, threads):
+    cpu = platform.processor() or platform.machine()
+    v, descr = _oracle_rate(algo, classes, lr, budget_s, threads)
+    return {"value": round(v, 4), "unit": "Mpix/s", "cores": threads, "kind": "port",
+            "sample": "sequential CPU oracle (one stream per thread, %d thread(s), %s) on the first frames of "
+                      "streams of each resolution class; per-class rates weighted by the workload's pixel mix: %s"
+                      % (threads, cpu, "; ".join(descr))}
 
+
+# ------------------------------------------------------------------------------------------------
+# batched workload
+# ------------------------------------------------------------------------------------------------
+def run_batch(args, preset, n_streams, ref_frames):
+    import torch
+    from litiv_amd import BackgroundSubtractorSuBSENSE, Batch, synth
+    from litiv_amd.autotune import tune_batch
+
+    steps = args.steps if args.steps is not None else ref_frames - args.warmup
+    total = args.warmup + steps
+    dev = torch.device("cuda:0")
+    cfgs = [synth.preset(preset, s) for s in range(n_streams)]
+    seqs = []
+    for c in cfgs:
+        c.frames = max(c.frames, total)
+        seqs.append(synth.Sequence(c))
+    px_step = sum(c.width * c.height for c in cfgs)
+    frames, masks = [], []
+    for s in seqs:
+        f = torch.empty(total * s.frame_bytes, dtype=torch.uint8, device=dev)
+        frames.append(f)
+        masks.append(torch.empty((2, s.cfg.width * s.cfg.height), dtype=torch.uint8, device=dev))
+    torch.cuda.synchronize()
+    for s, f in zip(seqs, frames):
+        s.frames_device(0, total, f.data_ptr())
+    torch.cuda.synchronize()
+    log("workload subsense_batch64: %d streams, %.2f Mpix per step, %d warm-up + %d timed steps, %.1f GB of frames"
+        % (n_streams, px_step / 1e6, args.warmup, steps, sum(f.numel() for f in frames) / 1e9))
+    f0 = [s.frame(0) for s in seqs]
+    subs = [BackgroundSubtractorSuBSENSE(stream_id=i, device=0) for i in range(n_streams)]
+
+    def init_all():
+        for b, f in zip(subs, f0):
+            b.initialize(f)
+
+    def ptrs(t):
+        return ([f.data_ptr() + t * s.frame_bytes for f, s in zip(frames, seqs)],
+                [m[t % 2].data_ptr() for m in masks])
+
+    # (1) the same streams serially through the plain drop-in: apply + synchronize per stream
+    plain = None
+    if not args.no_plain:
+        init_all()
+        for t in range(args.warmup):
+            im, mk = ptrs(t)
+            for i, b in enumerate(subs):
+                b.apply_device(im[i], mk[i])
+                b.synchronize()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for k in range(steps):
+            im, mk = ptrs(args.warmup + k)
+            for i, b in enumerate(subs):
+                b.apply_device(im[i], mk[i])
+                b.synchronize()
+        torch.cuda.synchronize()
+        wall_plain = time.perf_counter() - t0
+        plain = px_step * steps / wall_plain / 1e6
+        log("plain serial: %.3f ms/step, %.1f Mpix/s" % (wall_plain / steps * 1e3, plain))
+
+    # (2) the batch: warm-up steps double as the autotuner's trials (options never change results)
+    init_all()
+    batch = Batch(subs)
+    tuned = tune_batch(batch, lambda t: ptrs(t), args.warmup, log=log) if not args.no_tune else batch.options()
+    batch.synchronize()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(steps):
+        batch.apply_device(*ptrs(args.warmup + k))
+    batch.synchronize()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    mpix = px_step * steps / wall / 1e6
+    log("batched: %.3f ms/step, %.1f Mpix/s (options %s)" % (wall / steps * 1e3, mpix, tuned))
+    batch.close()
+
+    # (3) measurement passes over the same frames (re-initialized): match-kernel time with HIP events
+    # and exact examined-sample counters per stream, then whole-step latency
+    n_meas = min(steps, args.meas_steps)
+    init_all()
+    batch = Batch(subs, **tuned)
+    for t in range(args.warmup):
+        batch.apply_device(*ptrs(t))
+    batch.set_timing("match")
+    match_ms, mbytes, sbytes = [], 0.0, 0.0
+    iters, work = [], np.zeros(8)
+    for k in range(n_meas):
+        batch.apply_device(*ptrs(args.warmup + k))
+        ms, _ = batch.take_timing()
+        match_ms.append(ms)
+        for i, b in enumerate(subs):
+            st = b.stats()
+            C = st["channels"]
+            mbytes += match_bytes(C, st["n_roi"], st["examined_speculative"])
+            sbytes += step_bytes(C, st["n_roi"], st["examined"], st["writes"])
+            iters.append(st["fixpoint_iters"])
+            work += np.array(st["fixpoint_work"], dtype=np.float64)
+    batch.set_timing("")
+    batch.close()
+    init_all()
+    batch = Batch(subs, **tuned)
+    for t in range(args.warmup):
+        batch.apply_device(*ptrs(t))
+    batch.set_timing("step")
+    step_ms = []
+    for k in range(n_meas):
+        batch.apply_device(*ptrs(args.warmup + k))
+        ms, _ = batch.take_timing()
+        step_ms.append(ms)
+    batch.set_timing("")
+    batch.close()
+    avg_match_s = float(np.mean(match_ms)) / 1e3
+    bytes_launch = mbytes / n_meas
+    achieved = bytes_launch / avg_match_s / 1e9
+    result = {
+        "metric": METRIC,
+        "value": round(mpix, 2),
+        "unit": "Mpix/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall / steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u8/u16 integer model, f32 feedback maps",
+        "data": "synthetic (deterministic generator, per-stream seeds, frames resident in HBM)",
+        "config": {"workload": "subsense_batch64", "streams": n_streams,
+                   "mix": "16x320x240x3, 16x640x480x3, 16x640x512x1, 16x1920x1080x3",
+                   "pixels_per_step": px_step, "frames": total, "algo": "subsense", "learning_rate": 0.0,
+                   "launch": "one launch per kernel for all streams, hipGraph replay", "options": tuned,
+                   "parallelism": "64 streams batched on one GPU"},
+        "plain_serial": None if plain is None else {
+            "value": round(plain, 2), "unit": "Mpix/s",
+            "what": "same 64 streams and frames through the one-stream drop-in, apply + synchronize per stream"},
+        "batched_over_plain": None if plain is None else round(mpix / plain, 3),
+        "roofline": {"bound": "hbm", "kernel": "kb_match (speculative ordered sample match, all streams)",
+                     "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "bytes_per_launch": int(bytes_launch), "avg_launch_us": round(avg_match_s * 1e6, 2)},
+        "frame_latency_ms": {"p50": round(float(np.percentile(step_ms, 50)), 4),
+                             "p99": round(float(np.percentile(step_ms, 99)), 4),
+                             "what": "GPU time of one batched step (every stream's frame), HIP events, %d steps"
+                                     % n_meas},
+        "step_algorithmic_gbs": round(sbytes / n_meas / (float(np.mean(step_ms)) / 1e3) / 1e9, 1),
+        "fixpoint": {"mean_iters": round(float(np.mean(iters)), 3),
+                     "mean_worklist_per_stream": [round(float(x), 1) for x in (work / len(iters))[1:6]]},
+    }
+    if not args.no_cpu:
+        # one stream of each resolution class per thread (streams 16c .. 16c+threads-1), first frames
+        threads = max(1, min(args.cpu_threads, 16))
+        n_cpu = 24
+        classes = []
+        for g in range(4):
+            ids = list(range(16 * g, 16 * g + threads))
+            streams = [(i, [seqs[i].frame(t) for t in range(n_cpu)]) for i in ids]
+            weight = sum(cfgs[i].width * cfgs[i].height for i in range(16 * g, 16 * g + 16))
+            classes.append((weight, streams))
+        result["cpu_baseline"] = cpu_baseline("subsense", classes, 0.0, args.cpu_budget, threads)
+        result["cpu_baseline"]["ratio_gpu_over_cpu"] = round(mpix / result["cpu_baseline"]["value"], 1)
+    return result
+
+
+# ------------------------------------------------------------------------------------------------
+# plain single-stream workloads
+# ------------------------------------------------------------------------------------------------
+def run_plain(args, name, algo, preset, ref_frames):
     import torch
     from litiv_amd import BackgroundSubtractorLOBSTER, BackgroundSubtractorSuBSENSE, synth
 
-    algo, preset, over, ref_frames = WORKLOADS[args.workload]
     steps = args.steps if args.steps is not None else ref_frames - args.warmup
-    if args.profile_steps:
-        steps = min(steps, args.profile_steps)
     total = args.warmup + steps
-    cfg = synth.preset(preset, 0, **over)
+    cfg = synth.preset(preset, 0)
     cfg.frames = max(cfg.frames, total)
     seq = synth.Sequence(cfg)
     fb = seq.frame_bytes
@@ -103,23 +292,24 @@ def main():
     lr = 0.0 if algo == "subsense" else 16.0
     dev = torch.device("cuda:0")
     log("workload %s: %dx%dx%d, %d warm-up + %d timed frames, %.2f GB of frames in HBM" % (
-        args.workload, W, H, C, args.warmup, steps, total * fb / 1e9))
+        name, W, H, C, args.warmup, steps, total * fb / 1e9))
     frames = torch.empty(total * fb, dtype=torch.uint8, device=dev)
     masks = torch.empty((8, H * W), dtype=torch.uint8, device=dev)
     torch.cuda.synchronize()
     seq.frames_device(0, total, frames.data_ptr())
     torch.cuda.synchronize()
     base = frames.data_ptr()
-
     bgs = (BackgroundSubtractorSuBSENSE if algo == "subsense" else BackgroundSubtractorLOBSTER)(device=0)
-    f0 = frames[:fb].cpu().numpy().reshape((H, W, C) if C > 1 else (H, W))
-    bgs.initialize(f0)
-    for t in range(args.warmup):
-        bgs.apply_device(base + t * fb, masks[t % 8].data_ptr(), lr)
-    bgs.synchronize()
-    torch.cuda.synchronize()
+    f0 = seq.frame(0)
 
-    # timed region: K steps, device synchronized on both sides
+    def warm():
+        bgs.initialize(f0)
+        for t in range(args.warmup):
+            bgs.apply_device(base + t * fb, masks[t % 8].data_ptr(), lr)
+        bgs.synchronize()
+
+    warm()
+    torch.cuda.synchronize()
     t0 = time.perf_counter()
     for i in range(steps):
         t = args.warmup + i
@@ -130,23 +320,15 @@ def main():
     mpix = W * H * steps / wall / 1e6
     log("timed: %.3f s, %.3f ms/frame, %.1f Mpix/s" % (wall, wall / steps * 1e3, mpix))
 
-    # live per-kernel timing of the dominant kernel (match) and per-step latency, on the library's
-    # stream, over a second pass that continues the same sequence from the last timed state
-    # (re-initialized so the pass replays the timed frames exactly)
-    bgs.initialize(f0)
-    for t in range(args.warmup):
-        bgs.apply_device(base + t * fb, masks[t % 8].data_ptr(), lr)
-    bgs.synchronize()
-    n_meas = min(steps, 200)
+    n_meas = min(steps, args.meas_steps)
+    warm()
     ex_spec = ex_fin = wr = 0
-    iters, max_iter = [], 0
-    work = np.zeros(8)
-    match_ms = []
+    iters, max_iter, work, match_ms = [], 0, np.zeros(8), []
     bgs.set_timing("match")
     for i in range(n_meas):
         t = args.warmup + i
         bgs.apply_device(base + t * fb, masks[t % 8].data_ptr(), lr)
-        ms, cnt = bgs.take_timing()
+        ms, _ = bgs.take_timing()
         match_ms.append(ms)
         st = bgs.stats()
         ex_spec += st["examined_speculative"]
@@ -156,24 +338,21 @@ def main():
         work += np.array(st["fixpoint_work"], dtype=np.float64)
         max_iter = max(max_iter, st["max_fixpoint_iter"])
     bgs.set_timing("step")
-    bgs.initialize(f0)
-    for t in range(args.warmup):
-        bgs.apply_device(base + t * fb, masks[t % 8].data_ptr(), lr)
-    bgs.synchronize()
+    warm()
     step_ms = []
     for i in range(n_meas):
         t = args.warmup + i
         bgs.apply_device(base + t * fb, masks[t % 8].data_ptr(), lr)
-        ms, cnt = bgs.take_timing()
+        ms, _ = bgs.take_timing()
         step_ms.append(ms)
     bgs.set_timing("")
     n_roi = st["n_roi"]
     avg_match_s = float(np.mean(match_ms)) / 1e3
-    bytes_match = match_bytes_per_launch(C, n_roi, ex_spec / n_meas)
+    bytes_match = match_bytes(C, n_roi, ex_spec / n_meas)
     achieved = bytes_match / avg_match_s / 1e9
-    step_bytes = step_bytes_per_frame(C, n_roi, ex_fin / n_meas, wr / n_meas)
+    sb = step_bytes(C, n_roi, ex_fin / n_meas, wr / n_meas)
     result = {
-        "metric": "SuBSENSE Mpix/s: 64-stream batched vs 1-stream plain; p99 frame latency",
+        "metric": METRIC,
         "value": round(mpix, 2),
         "unit": "Mpix/s",
         "n_gpus": 1,
@@ -185,7 +364,7 @@ def main():
         "vs_baseline": None,
         "dtype": "u8/u16 integer model, f32 feedback maps",
         "data": "synthetic (deterministic generator, frames resident in HBM)",
-        "config": {"workload": args.workload, "width": W, "height": H, "channels": C, "frames": total,
+        "config": {"workload": name, "width": W, "height": H, "channels": C, "frames": total,
                    "algo": algo, "learning_rate": lr, "streams": 1, "parallelism": "single stream, one GPU"},
         "roofline": {"bound": "hbm", "kernel": "k_match (speculative ordered sample match)",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -195,15 +374,40 @@ def main():
         "frame_latency_ms": {"p50": round(float(np.percentile(step_ms, 50)), 4),
                              "p99": round(float(np.percentile(step_ms, 99)), 4),
                              "what": "GPU time of one frame step (HIP events), %d frames" % n_meas},
-        "step_algorithmic_gbs": round(step_bytes / (float(np.mean(step_ms)) / 1e3) / 1e9, 1),
+        "step_algorithmic_gbs": round(sb / (float(np.mean(step_ms)) / 1e3) / 1e9, 1),
         "fixpoint": {"mean_iters": round(float(np.mean(iters)), 3), "max_iter": int(max_iter),
                      "mean_worklist": [round(float(x), 1) for x in (work / n_meas)[1:6]]},
     }
     if not args.no_cpu:
         n_cpu = 40
-        host = frames[: n_cpu * fb].cpu().numpy().reshape((n_cpu, H, W, C) if C > 1 else (n_cpu, H, W))
-        result["cpu_baseline"] = cpu_baseline(algo, list(host), lr, args.cpu_budget)
+        streams = [(0, [seq.frame(t) for t in range(n_cpu)])]
+        result["cpu_baseline"] = cpu_baseline(algo, [(W * H, streams)], lr, args.cpu_budget, 1)
         result["cpu_baseline"]["ratio_gpu_over_cpu"] = round(mpix / result["cpu_baseline"]["value"], 1)
+    return result
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=None)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--workload", default="subsense_batch64", choices=sorted(WORKLOADS))
+    ap.add_argument("--cpu-budget", type=float, default=16.0, help="seconds of oracle time for the CPU baseline")
+    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--meas-steps", type=int, default=60, help="steps of each measurement pass")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true", help="batched workload: skip the serial plain comparison")
+    ap.add_argument("--no-tune", action="store_true", help="batched workload: default launch options")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU; multi-GPU runs (--gpus %d) are not part of it. Streams "
+              "shard across GPUs with no exchange between them." % args.gpus, file=sys.stderr)
+        sys.exit(2)
+    kind, algo, preset, n_streams, ref_frames = WORKLOADS[args.workload]
+    if kind == "batch":
+        result = run_batch(args, preset, n_streams, ref_frames)
+    else:
+        result = run_plain(args, args.workload, algo, preset, ref_frames)
     print(json.dumps(result), flush=True)
 
 

@@ -158,15 +158,26 @@ int lvbgs_batch_create(lvbgs_ctx* const* ctxs, int n, lvbgs_batch** out);
 void lvbgs_batch_destroy(lvbgs_batch* b);
 const char* lvbgs_batch_last_error(const lvbgs_batch* b);
 void* lvbgs_batch_stream(lvbgs_batch* b);
-/* use_graph: capture the step once as a hipGraph and replay it; schedule knobs as for single contexts */
-int lvbgs_batch_set_options(lvbgs_batch* b, int use_graph, int seq_limit, int cont_blocks, int fixpoint_launches,
-                            int fixpoint_blocks);
+/* Launch options of a batch. Results never depend on them. */
+typedef struct lvbgs_batch_options {
+    int use_graph;          /* capture the step once as a hipGraph and replay it */
+    int seq_limit;          /* samples scanned by the sequential match phase (1..255) */
+    int cont_blocks;        /* cooperative match phase: grid of a stream, at most */
+    int cont_px_per_block;  /*   ... and ceil(pixels / this) when > 0 */
+    int fixpoint_launches;  /* full-grid fix-point iterations before the one-workgroup tail */
+    int fixpoint_blocks;    /* fix-point grid of a stream, at most */
+    int fix_px_per_block;   /*   ... and ceil(pixels / this) when > 0 */
+} lvbgs_batch_options;
+int lvbgs_batch_default_options(lvbgs_batch_options* out);
+int lvbgs_batch_get_options(const lvbgs_batch* b, lvbgs_batch_options* out);
+int lvbgs_batch_set_options(lvbgs_batch* b, const lvbgs_batch_options* opts);
 /* one apply() of every stream: d_imgs[i] is stream i's frame, d_masks (nullable) its output mask,
  * lrs (nullable = 0) its learning-rate override */
 int lvbgs_batch_apply_device(lvbgs_batch* b, const void* const* d_imgs, void* const* d_masks, const double* lrs);
 int lvbgs_batch_synchronize(lvbgs_batch* b);
-/* per-step GPU time of batch steps (HIP events around each step on the batch stream) */
-int lvbgs_batch_set_timing(lvbgs_batch* b, int enabled);
+/* GPU time of batch steps: stage "match" (the match kernels) | "step" (whole step) | "" (off);
+ * HIP events on the batch stream, summed by lvbgs_batch_take_timing (which synchronizes) */
+int lvbgs_batch_set_timing(lvbgs_batch* b, const char* stage);
 int lvbgs_batch_take_timing(lvbgs_batch* b, double* total_ms, int* count);
 
 #ifdef __cplusplus

@@ -56,6 +56,12 @@ class Stats(C.Structure):
                 ("fixpoint_work", C.c_uint32 * 8)]
 
 
+class BatchOptions(C.Structure):
+    _fields_ = [("use_graph", C.c_int), ("seq_limit", C.c_int), ("cont_blocks", C.c_int),
+                ("cont_px_per_block", C.c_int), ("fixpoint_launches", C.c_int), ("fixpoint_blocks", C.c_int),
+                ("fix_px_per_block", C.c_int)]
+
+
 _lib = None
 
 
@@ -108,10 +114,12 @@ def lib():
         L.lvbgs_batch_last_error.restype = C.c_char_p
         L.lvbgs_batch_stream.argtypes = [vp]
         L.lvbgs_batch_stream.restype = vp
-        L.lvbgs_batch_set_options.argtypes = [vp, i, i, i, i, i]
+        L.lvbgs_batch_default_options.argtypes = [C.POINTER(BatchOptions)]
+        L.lvbgs_batch_get_options.argtypes = [vp, C.POINTER(BatchOptions)]
+        L.lvbgs_batch_set_options.argtypes = [vp, C.POINTER(BatchOptions)]
         L.lvbgs_batch_apply_device.argtypes = [vp, C.POINTER(vp), C.POINTER(vp), C.POINTER(C.c_double)]
         L.lvbgs_batch_synchronize.argtypes = [vp]
-        L.lvbgs_batch_set_timing.argtypes = [vp, i]
+        L.lvbgs_batch_set_timing.argtypes = [vp, C.c_char_p]
         L.lvbgs_batch_take_timing.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int)]
         _lib = L
     return _lib
@@ -318,8 +326,7 @@ class Batch:
     of stepping each subtractor alone. While the batch exists its members are stepped only through it;
     getters and state reads of a member are valid after synchronize()."""
 
-    def __init__(self, subtractors, use_graph=True, seq_limit=255, cont_blocks=1024, fixpoint_launches=3,
-                 fixpoint_blocks=256):
+    def __init__(self, subtractors, **options):
         self.members = list(subtractors)
         hs = (C.c_void_p * len(self.members))(*[m._h.value for m in self.members])
         h = C.c_void_p()
@@ -331,7 +338,8 @@ class Batch:
         self._imgs = (C.c_void_p * len(self.members))()
         self._masks = (C.c_void_p * len(self.members))()
         self._lrs = (C.c_double * len(self.members))()
-        self.set_options(use_graph, seq_limit, cont_blocks, fixpoint_launches, fixpoint_blocks)
+        if options:
+            self.set_options(**options)
 
     def close(self):
         if getattr(self, "_h", None):
@@ -346,9 +354,21 @@ class Batch:
             raise LvbgsError(rc, lib().lvbgs_batch_last_error(self._h).decode())
         return rc
 
-    def set_options(self, use_graph=True, seq_limit=255, cont_blocks=1024, fixpoint_launches=3, fixpoint_blocks=256):
-        self._check(lib().lvbgs_batch_set_options(self._h, int(bool(use_graph)), int(seq_limit), int(cont_blocks),
-                                                  int(fixpoint_launches), int(fixpoint_blocks)))
+    def options(self):
+        o = BatchOptions()
+        self._check(lib().lvbgs_batch_get_options(self._h, C.byref(o)))
+        return {k: getattr(o, k) for k, _ in BatchOptions._fields_}
+
+    def set_options(self, **kw):
+        """Launch options (use_graph, seq_limit, cont_blocks, cont_px_per_block, fixpoint_launches,
+        fixpoint_blocks, fix_px_per_block); unspecified ones keep their current value."""
+        o = BatchOptions()
+        self._check(lib().lvbgs_batch_get_options(self._h, C.byref(o)))
+        for k, v in kw.items():
+            if not hasattr(o, k):
+                raise TypeError("unknown batch option %r" % k)
+            setattr(o, k, int(v))
+        self._check(lib().lvbgs_batch_set_options(self._h, C.byref(o)))
 
     def apply_device(self, d_imgs, d_masks=None, learningRates=None):
         """One apply() of every member: device frame pointers (and optional mask pointers) in member order."""
@@ -367,8 +387,9 @@ class Batch:
     def stream(self):
         return lib().lvbgs_batch_stream(self._h)
 
-    def set_timing(self, enabled):
-        self._check(lib().lvbgs_batch_set_timing(self._h, int(bool(enabled))))
+    def set_timing(self, stage):
+        """"match" | "step" | "" (off): HIP-event timing of each following step."""
+        self._check(lib().lvbgs_batch_set_timing(self._h, (stage or "").encode()))
 
     def take_timing(self):
         ms, n = C.c_double(), C.c_int()

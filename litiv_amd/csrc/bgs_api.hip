@@ -781,7 +781,17 @@ extern "C" int lvbgs_batch_create(lvbgs_ctx* const* ctxs, int n, lvbgs_batch** o
     lvbgs_batch* b = new lvbgs_batch();
     b->ctx.assign(ctxs, ctxs + n);
     b->device = ctxs[0]->device;
-    b->opt = ctxs[0]->opt;
+    {
+        lvbgs_batch_options d;
+        lvbgs_batch_default_options(&d);
+        b->use_graph = d.use_graph != 0;
+        b->opt.seq_limit = d.seq_limit;
+        b->opt.cont_blocks = d.cont_blocks;
+        b->opt.cont_px_per_block = d.cont_px_per_block;
+        b->opt.fixpoint_launches = d.fixpoint_launches;
+        b->opt.fixpoint_blocks = d.fixpoint_blocks;
+        b->opt.fix_px_per_block = d.fix_px_per_block;
+    }
     hipSetDevice(b->device);
     int rc = LVBGS_OK;
     if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
@@ -814,18 +824,45 @@ extern "C" void lvbgs_batch_destroy(lvbgs_batch* b) {
 extern "C" const char* lvbgs_batch_last_error(const lvbgs_batch* b) { return b ? b->err.c_str() : "null batch"; }
 extern "C" void* lvbgs_batch_stream(lvbgs_batch* b) { return b ? (void*)b->stream : nullptr; }
 
-extern "C" int lvbgs_batch_set_options(lvbgs_batch* b, int use_graph, int seq_limit, int cont_blocks,
-                                       int fixpoint_launches, int fixpoint_blocks) {
-    if (!b || seq_limit < 1 || seq_limit > 255 || cont_blocks < 1 || cont_blocks > 65535 || fixpoint_launches < 0 ||
-        fixpoint_launches > 64 || fixpoint_blocks < 1 || fixpoint_blocks > 65535)
-        return LVBGS_ERR_ARG;
+extern "C" int lvbgs_batch_default_options(lvbgs_batch_options* o) {
+    if (!o) return LVBGS_ERR_ARG;
+    o->use_graph = 1;
+    o->seq_limit = 255;
+    o->cont_blocks = 1024;
+    o->cont_px_per_block = 2048;
+    o->fixpoint_launches = 3;
+    o->fixpoint_blocks = 256;
+    o->fix_px_per_block = 8192;
+    return LVBGS_OK;
+}
+
+extern "C" int lvbgs_batch_get_options(const lvbgs_batch* b, lvbgs_batch_options* o) {
+    if (!b || !o) return LVBGS_ERR_ARG;
+    o->use_graph = b->use_graph ? 1 : 0;
+    o->seq_limit = b->opt.seq_limit;
+    o->cont_blocks = b->opt.cont_blocks;
+    o->cont_px_per_block = b->opt.cont_px_per_block;
+    o->fixpoint_launches = b->opt.fixpoint_launches;
+    o->fixpoint_blocks = b->opt.fixpoint_blocks;
+    o->fix_px_per_block = b->opt.fix_px_per_block;
+    return LVBGS_OK;
+}
+
+extern "C" int lvbgs_batch_set_options(lvbgs_batch* b, const lvbgs_batch_options* o) {
+    if (!b || !o) return LVBGS_ERR_ARG;
+    if (o->seq_limit < 1 || o->seq_limit > 255 || o->cont_blocks < 1 || o->cont_blocks > 65535 ||
+        o->fixpoint_launches < 0 || o->fixpoint_launches > 64 || o->fixpoint_blocks < 1 || o->fixpoint_blocks > 65535 ||
+        o->cont_px_per_block < 0 || o->fix_px_per_block < 0)
+        return bset_err(b, LVBGS_ERR_ARG, "invalid batch options");
     hipSetDevice(b->device);
     hipStreamSynchronize(b->stream);
-    b->use_graph = use_graph != 0;
-    b->opt.seq_limit = seq_limit;
-    b->opt.cont_blocks = cont_blocks;
-    b->opt.fixpoint_launches = fixpoint_launches;
-    b->opt.fixpoint_blocks = fixpoint_blocks;
+    b->use_graph = o->use_graph != 0;
+    b->opt.seq_limit = o->seq_limit;
+    b->opt.cont_blocks = o->cont_blocks;
+    b->opt.cont_px_per_block = o->cont_px_per_block;
+    b->opt.fixpoint_launches = o->fixpoint_launches;
+    b->opt.fixpoint_blocks = o->fixpoint_blocks;
+    b->opt.fix_px_per_block = o->fix_px_per_block;
     return batch_plan(b);
 }
 
@@ -851,6 +888,8 @@ extern "C" int lvbgs_batch_apply_device(lvbgs_batch* b, const void* const* d_img
         hipEventRecord(b->ev_copy[i], b->stream) != hipSuccess)
         return bset_err(b, LVBGS_ERR_HIP, "view upload");
     b->ev_live[i] = true;
+    StepOptions o = b->opt;
+    o.timed_stage = 0;
     if (b->timed) {
         if (b->ev_used == b->ev_pairs.size()) {
             hipEvent_t e0, e1;
@@ -858,9 +897,16 @@ extern "C" int lvbgs_batch_apply_device(lvbgs_batch* b, const void* const* d_img
             hipEventCreate(&e1);
             b->ev_pairs.emplace_back(e0, e1);
         }
-        (void)hipEventRecord(b->ev_pairs[b->ev_used].first, b->stream);
+        o.ev_begin = b->ev_pairs[b->ev_used].first;
+        o.ev_end = b->ev_pairs[b->ev_used].second;
+        ++b->ev_used;
     }
-    if (b->use_graph) {
+    if (b->timed == 1) {
+        // kernel-stage timing: events between the launches, so the step is issued directly
+        o.timed_stage = 1;
+        batch_step(b->d_views, ns, b->prefix, b->total, o, b->any_cont, b->stream);
+    } else if (b->use_graph) {
+        if (b->timed == 2) (void)hipEventRecord(o.ev_begin, b->stream);
         if (!b->graph) {
             hipGraph_t g = nullptr;
             if (hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
@@ -876,10 +922,11 @@ extern "C" int lvbgs_batch_apply_device(lvbgs_batch* b, const void* const* d_img
             }
         }
         if (hipGraphLaunch(b->graph, b->stream) != hipSuccess) return bset_err(b, LVBGS_ERR_HIP, "graph launch");
+        if (b->timed == 2) (void)hipEventRecord(o.ev_end, b->stream);
     } else {
-        batch_step(b->d_views, ns, b->prefix, b->total, b->opt, b->any_cont, b->stream);
+        o.timed_stage = b->timed;
+        batch_step(b->d_views, ns, b->prefix, b->total, o, b->any_cont, b->stream);
     }
-    if (b->timed) (void)hipEventRecord(b->ev_pairs[b->ev_used++].second, b->stream);
     return hipGetLastError() == hipSuccess ? LVBGS_OK : bset_err(b, LVBGS_ERR_HIP, "batch launch");
 }
 
@@ -896,9 +943,15 @@ extern "C" int lvbgs_batch_synchronize(lvbgs_batch* b) {
     return LVBGS_OK;
 }
 
-extern "C" int lvbgs_batch_set_timing(lvbgs_batch* b, int enabled) {
+extern "C" int lvbgs_batch_set_timing(lvbgs_batch* b, const char* stage) {
     if (!b) return LVBGS_ERR_ARG;
-    b->timed = enabled ? 1 : 0;
+    const std::string s = stage ? stage : "";
+    if (s.empty()) b->timed = 0;
+    else if (s == "match") b->timed = 1;
+    else if (s == "step") b->timed = 2;
+    else return bset_err(b, LVBGS_ERR_ARG, "timing stage must be \"match\", \"step\" or \"\"");
+    hipSetDevice(b->device);
+    hipStreamSynchronize(b->stream);
     b->ev_used = 0;
     return LVBGS_OK;
 }

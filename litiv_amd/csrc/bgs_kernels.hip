@@ -22,6 +22,8 @@
 
 #include <cmath>
 
+#include <algorithm>
+
 #include "bgs_kernels.hpp"
 
 namespace lvbgs {
@@ -1716,7 +1718,9 @@ void kernel_grids(const View& v, const StepOptions& opt, uint32_t g[kNumGrids]) 
     g[kGridCell] = v.lr_scaling ? cdiv((size_t)v.dsW * v.dsH, TPB) : 0u;
     g[kGridOne] = 1u;
     g[kGridFix] = (uint32_t)opt.fixpoint_blocks;
+    if (opt.fix_px_per_block > 0) g[kGridFix] = std::min(g[kGridFix], cdiv(v.npx, (size_t)opt.fix_px_per_block));
     g[kGridCont] = (uint32_t)opt.cont_blocks;
+    if (opt.cont_px_per_block > 0) g[kGridCont] = std::min(g[kGridCont], cdiv(v.npx, (size_t)opt.cont_px_per_block));
 }
 
 template <int C, int ALGO>
@@ -1771,9 +1775,12 @@ void frame_step(const View& v, const StepOptions& opt, hipStream_t st) {
 void batch_step(const View* d_views, int ns, const uint32_t* const prefix[kNumGrids], const uint32_t total[kNumGrids],
                 const StepOptions& opt, bool any_cont, hipStream_t st) {
     auto args = [&](int kind) { return BatchArgs{d_views, prefix[kind], ns}; };
+    if (opt.timed_stage == 2) (void)hipEventRecord(opt.ev_begin, st);
     hipLaunchKernelGGL(kb_describe, dim3(total[kGridTile]), dim3(TPB), 0, st, args(kGridTile));
+    if (opt.timed_stage == 1) (void)hipEventRecord(opt.ev_begin, st);
     hipLaunchKernelGGL(kb_match, dim3(total[kGridTile]), dim3(TPB), 0, st, args(kGridTile));
     if (any_cont) hipLaunchKernelGGL(kb_match_cont, dim3(total[kGridCont]), dim3(TPB), 0, st, args(kGridCont));
+    if (opt.timed_stage == 1) (void)hipEventRecord(opt.ev_end, st);
     for (uint32_t k = 1; k <= (uint32_t)opt.fixpoint_launches; ++k)
         hipLaunchKernelGGL(kb_fixpoint, dim3(total[kGridFix]), dim3(TPB), 0, st, args(kGridFix), k);
     hipLaunchKernelGGL(kb_fixpoint_tail, dim3(total[kGridOne]), dim3(256), 0, st, args(kGridOne),
@@ -1789,6 +1796,7 @@ void batch_step(const View* d_views, int ns, const uint32_t* const prefix[kNumGr
     if (total[kGridCell]) hipLaunchKernelGGL(kb_downsample, dim3(total[kGridCell]), dim3(TPB), 0, st, args(kGridCell));
     hipLaunchKernelGGL(kb_decide, dim3(total[kGridOne]), dim3(256), 0, st, args(kGridOne));
     hipLaunchKernelGGL(kb_refresh, dim3(total[kGridPix]), dim3(TPB), 0, st, args(kGridPix));
+    if (opt.timed_stage == 2) (void)hipEventRecord(opt.ev_end, st);
 }
 
 void launch_refresh(const View& v, float frac, int force_fg, uint32_t frame, uint32_t ev, hipStream_t st) {

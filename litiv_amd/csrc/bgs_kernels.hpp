@@ -12,6 +12,8 @@ struct StepOptions {
     int tail_cap = 1 << 16;      // iteration cap of the single-workgroup tail (error flag if reached)
     int seq_limit = 255;         // samples scanned per pixel by the sequential phase of the match
     int cont_blocks = 1024;      // grid of the cooperative phase (grid-stride over unfinished pixels)
+    int fix_px_per_block = 0;    // > 0: fix-point grid of a stream = ceil(npx / this), at most fixpoint_blocks
+    int cont_px_per_block = 0;   // > 0: cooperative grid of a stream = ceil(npx / this), at most cont_blocks
     int timed_stage = 0;         // 0: none, 1: the match kernel, 2: the whole step
     hipEvent_t ev_begin = nullptr, ev_end = nullptr;
 };

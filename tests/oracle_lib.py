@@ -135,8 +135,13 @@ class Oracle:
         if lr is None:
             lr = 0.0 if self.algo == SUBSENSE else 16.0
         img = np.ascontiguousarray(img, dtype=np.uint8)
-        h, w, _ = self.shape
-        mask = np.zeros((h, w), np.uint8)
+        if self.shape is None:
+            mask = np.zeros(1, np.uint8)  # orc_apply rejects the call before touching the mask
+        else:
+            h, w, c = self.shape
+            if img.shape[:2] != (h, w) or (1 if img.ndim == 2 else img.shape[2]) != c:
+                raise RuntimeError("input image type/size mismatch with initialization type/size")
+            mask = np.zeros((h, w), np.uint8)
         rc = lib().orc_apply(self.h, img, mask, float(lr))
         if rc != 0:
             raise RuntimeError(self._err())

@@ -1,0 +1,89 @@
+"""C ABI checks that need no GPU: every function declared in include/*.h is exported by the built
+libraries, the host-only entry points behave as the reference's (default parameters, validateROI),
+the synthetic generator is deterministic and matches its configuration, and bench.py refuses
+multi-GPU runs. No call here launches device work."""
+import ctypes as C
+import os
+import re
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADERS = {"litiv_bgs.h": "liblitiv_bgs.so", "litiv_synth.h": "liblitiv_synth.so"}
+
+
+def _declared(header):
+    text = open(os.path.join(ROOT, "include", header)).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(lv(?:bgs|synth)_\w+)\s*\(", text)))
+
+
+@pytest.mark.parametrize("header", sorted(HEADERS))
+def test_every_declared_function_is_exported(header):
+    lib = C.CDLL(os.path.join(ROOT, "litiv_amd", HEADERS[header]))
+    names = _declared(header)
+    assert len(names) >= 4
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def _bgs():
+    from litiv_amd import bgs
+    return bgs
+
+
+def test_default_parameters_are_the_reference_defaults():
+    bgs = _bgs()
+    s = bgs.default_params(bgs.SUBSENSE)  # BackgroundSubtractorSuBSENSE.hpp:22-31
+    assert (s.desc_dist_threshold, s.color_dist_threshold, s.bg_samples, s.required_bg_samples,
+            s.samples_for_mv_avgs, s.lbsp_threshold_offset, s.median_blur_kernel) == (3, 30, 50, 2, 100, 0, 9)
+    assert abs(s.rel_lbsp_threshold - 0.333) < 1e-7
+    lo = bgs.default_params(bgs.LOBSTER)  # BackgroundSubtractorLOBSTER.hpp:22-31
+    assert (lo.desc_dist_threshold, lo.color_dist_threshold, lo.bg_samples, lo.required_bg_samples) == (4, 30, 35, 2)
+    with pytest.raises(ValueError):
+        bgs.default_params(7)
+
+
+def test_validate_roi_clears_the_lbsp_border():
+    bgs = _bgs()
+    roi = np.full((12, 17), 255, np.uint8)
+    out = bgs.BackgroundSubtractor.validateROI(roi)
+    assert out[2:10, 2:15].all() and not out[:2].any() and not out[10:].any()
+    assert not out[:, :2].any() and not out[:, 15:].any()
+    assert roi.all()  # the input is not modified (returns a validated copy)
+
+
+def test_synthetic_generator_is_deterministic_and_configured():
+    from litiv_amd import synth
+    cfg = synth.preset(synth.QVGA, 0)
+    assert (cfg.width, cfg.height, cfg.channels, cfg.frames, cfg.n_objects) == (320, 240, 3, 300, 3)
+    a = synth.Sequence(cfg)
+    b = synth.Sequence(synth.preset(synth.QVGA, 0))
+    for t in (0, 7, 150):
+        assert np.array_equal(a.frame(t), b.frame(t))
+    assert a.frame(0).shape == (240, 320, 3)
+    assert not np.array_equal(a.frame(0), a.frame(5))  # objects move
+    other = synth.Sequence(synth.preset(synth.QVGA, 1))
+    assert not np.array_equal(a.frame(0), other.frame(0))  # per-stream seed
+    # background without noise or objects stays in the value-noise range [40, 215]
+    quiet = synth.Sequence(synth.preset(synth.QVGA, 0, noise_sigma=0, n_objects=0))
+    f = quiet.frame(0).astype(int)
+    assert f.min() >= 40 - 10 and f.max() <= 215 + 10  # +-10 illumination ramp
+
+
+def test_batch64_mix():
+    from litiv_amd import synth
+    shapes = [(c.width, c.height, c.channels) for c in (synth.preset(synth.BATCH64, s) for s in range(64))]
+    for g, want in enumerate([(320, 240, 3), (640, 480, 3), (640, 512, 1), (1920, 1080, 3)]):
+        assert shapes[16 * g:16 * g + 16] == [want] * 16
+    with pytest.raises(ValueError):
+        synth.preset(synth.BATCH64, 64)
+
+
+def test_bench_refuses_multi_gpu():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2 and "one GPU" in r.stderr and r.stdout == ""

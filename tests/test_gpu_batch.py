@@ -92,7 +92,7 @@ def test_batch_then_single_steps_continue_the_sequence():
     m = torch.empty((96, 128), dtype=torch.uint8, device="cuda")
     m2 = torch.empty((96, 128), dtype=torch.uint8, device="cuda")
     torch.cuda.synchronize()
-    batch = Batch([a, b], use_graph=True, seq_limit=4, cont_blocks=64)
+    batch = Batch([a, b], use_graph=1, seq_limit=4, cont_blocks=64, cont_px_per_block=512, fix_px_per_block=1024)
     for t in range(8):
         batch.apply_device([d[t].data_ptr(), d[t].data_ptr()], [m.data_ptr(), m2.data_ptr()])
         batch.synchronize()
