@@ -62,52 +62,63 @@ def step_bytes(C, n_roi, examined, writes):
 # ------------------------------------------------------------------------------------------------
 # CPU baseline: the sequential oracle, one stream per host thread (ctypes releases the GIL)
 # ------------------------------------------------------------------------------------------------
-def _oracle_rate(algo, classes, lr, budget_s, threads):
-    """classes: list of (weight_px_per_step, [frames of one stream per thread]). Each class runs with
-    `threads` threads (each on its own stream) for about budget_s/len(classes) seconds; returns the
-    mix rate (pixels of one workload step / CPU time of one step) and per-class rates."""
+def _oracle_rate(algo, classes, lr, budget_s):
+    """classes: list of (weight = pixels of the class per workload step, [(stream id, synth sequence,
+    frames to prepare)] with one stream per thread). Per class, every thread builds its oracle,
+    initializes it and generates its frames (untimed, in parallel), then all threads apply frames
+    concurrently for about budget_s/len(classes) seconds. Returns the mix rate (pixels of one workload
+    step / CPU wall time of one step) and a description per class."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from oracle_lib import LOBSTER, SUBSENSE, Oracle
     per_class = budget_s / len(classes)
     rates, descr = [], []
     for weight, streams in classes:
-        streams = streams[:threads]
-        orcs = []
-        for sid, frames in streams:
-            o = Oracle(SUBSENSE if algo == "subsense" else LOBSTER, stream_id=sid)
-            o.initialize(frames[0])
-            orcs.append(o)
-        done = [0] * len(streams)
-        px = [f[0].shape[0] * f[0].shape[1] for _, f in streams]
-        stop = time.perf_counter() + per_class
+        n = len(streams)
+        orcs, frames = [None] * n, [None] * n
+        done = [0] * n
+        ready = threading.Barrier(n + 1)
+        go = threading.Event()
+        stop = [0.0]
 
         def work(i):
-            frames = streams[i][1]
-            while time.perf_counter() < stop and done[i] < len(frames) - 1:
-                orcs[i].apply(frames[1 + done[i]], lr)
+            sid, seq, n_frames = streams[i]
+            frames[i] = [seq.frame(t) for t in range(n_frames)]
+            o = Oracle(SUBSENSE if algo == "subsense" else LOBSTER, stream_id=sid)
+            o.initialize(frames[i][0])
+            orcs[i] = o
+            ready.wait()
+            go.wait()
+            while time.perf_counter() < stop[0] and done[i] < n_frames - 1:
+                o.apply(frames[i][1 + done[i]], lr)
                 done[i] += 1
 
-        t0 = time.perf_counter()
-        ths = [threading.Thread(target=work, args=(i,)) for i in range(len(streams))]
+        ths = [threading.Thread(target=work, args=(i,)) for i in range(n)]
         for t in ths:
             t.start()
+        ready.wait()
+        t0 = time.perf_counter()
+        stop[0] = t0 + per_class
+        go.set()
         for t in ths:
             t.join()
         wall = time.perf_counter() - t0
-        rate = sum(px[i] * done[i] for i in range(len(streams))) / wall
+        f = frames[0][0]
+        rate = sum(f.shape[0] * f.shape[1] * d for d in done) / wall
         rates.append((weight, rate))
-        descr.append("%dx%d: %d frames over %d thread(s)" % (streams[0][1][0].shape[1], streams[0][1][0].shape[0],
-                                                               sum(done), len(streams)))
+        descr.append("%dx%d: %d frames over %d thread(s) in %.1f s" % (f.shape[1], f.shape[0], sum(done), n, wall))
+        log("cpu baseline: " + descr[-1])
+        del orcs, frames
     total_px = sum(w for w, _ in rates)
     step_s = sum(w / r for w, r in rates)
     return total_px / step_s / 1e6, descr
 
 
-def cpu_baseline(algo, classes, lr, budget_s, threads):
+def cpu_baseline(algo, classes, lr, budget_s):
     cpu = platform.processor() or platform.machine()
-    v, descr = _oracle_rate(algo, classes, lr, budget_s, threads)
+    threads = max(len(streams) for _, streams in classes)
+    v, descr = _oracle_rate(algo, classes, lr, budget_s)
     return {"value": round(v, 4), "unit": "Mpix/s", "cores": threads, "kind": "port",
-            "sample": "sequential CPU oracle (one stream per thread, %d thread(s), %s) on the first frames of "
+            "sample": "sequential CPU oracle, one stream per thread (%d thread(s), %s), on the first frames of "
                       "streams of each resolution class; per-class rates weighted by the workload's pixel mix: %s"
                       % (threads, cpu, "; ".join(descr))}
 
@@ -191,6 +202,7 @@ def run_batch(args, preset, n_streams, ref_frames):
     # (3) measurement passes over the same frames (re-initialized): match-kernel time with HIP events
     # and exact examined-sample counters per stream, then whole-step latency
     n_meas = min(steps, args.meas_steps)
+    log("measurement pass: match-kernel timing and counters over %d steps" % n_meas)
     init_all()
     batch = Batch(subs, **tuned)
     for t in range(args.warmup):
@@ -211,6 +223,7 @@ def run_batch(args, preset, n_streams, ref_frames):
             work += np.array(st["fixpoint_work"], dtype=np.float64)
     batch.set_timing("")
     batch.close()
+    log("measurement pass: step latency over %d steps" % n_meas)
     init_all()
     batch = Batch(subs, **tuned)
     for t in range(args.warmup):
@@ -223,6 +236,7 @@ def run_batch(args, preset, n_streams, ref_frames):
         step_ms.append(ms)
     batch.set_timing("")
     batch.close()
+    log("step latency p50 %.3f ms, p99 %.3f ms" % (np.percentile(step_ms, 50), np.percentile(step_ms, 99)))
     avg_match_s = float(np.mean(match_ms)) / 1e3
     bytes_launch = mbytes / n_meas
     achieved = bytes_launch / avg_match_s / 1e9
@@ -261,16 +275,16 @@ def run_batch(args, preset, n_streams, ref_frames):
                      "mean_worklist_per_stream": [round(float(x), 1) for x in (work / len(iters))[1:6]]},
     }
     if not args.no_cpu:
-        # one stream of each resolution class per thread (streams 16c .. 16c+threads-1), first frames
+        # streams 16g .. 16g+threads-1 of each resolution class g, one per thread, first frames
         threads = max(1, min(args.cpu_threads, 16))
-        n_cpu = 24
         classes = []
         for g in range(4):
-            ids = list(range(16 * g, 16 * g + threads))
-            streams = [(i, [seqs[i].frame(t) for t in range(n_cpu)]) for i in ids]
+            px = cfgs[16 * g].width * cfgs[16 * g].height
+            n_frames = max(4, min(40, int(2.5e6 * args.cpu_budget / 4 / px) + 2))
+            streams = [(i, seqs[i], n_frames) for i in range(16 * g, 16 * g + threads)]
             weight = sum(cfgs[i].width * cfgs[i].height for i in range(16 * g, 16 * g + 16))
             classes.append((weight, streams))
-        result["cpu_baseline"] = cpu_baseline("subsense", classes, 0.0, args.cpu_budget, threads)
+        result["cpu_baseline"] = cpu_baseline("subsense", classes, 0.0, args.cpu_budget)
         result["cpu_baseline"]["ratio_gpu_over_cpu"] = round(mpix / result["cpu_baseline"]["value"], 1)
     return result
 
@@ -379,9 +393,8 @@ def run_plain(args, name, algo, preset, ref_frames):
                      "mean_worklist": [round(float(x), 1) for x in (work / n_meas)[1:6]]},
     }
     if not args.no_cpu:
-        n_cpu = 40
-        streams = [(0, [seq.frame(t) for t in range(n_cpu)])]
-        result["cpu_baseline"] = cpu_baseline(algo, [(W * H, streams)], lr, args.cpu_budget, 1)
+        n_frames = max(4, min(60, int(3e6 * args.cpu_budget / (W * H)) + 2))
+        result["cpu_baseline"] = cpu_baseline(algo, [(W * H, [(0, seq, n_frames)])], lr, args.cpu_budget)
         result["cpu_baseline"]["ratio_gpu_over_cpu"] = round(mpix / result["cpu_baseline"]["value"], 1)
     return result
 
@@ -393,7 +406,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="subsense_batch64", choices=sorted(WORKLOADS))
     ap.add_argument("--cpu-budget", type=float, default=16.0, help="seconds of oracle time for the CPU baseline")
-    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1),
+                    help="batched workload: host threads of the CPU baseline (one stream each)")
     ap.add_argument("--meas-steps", type=int, default=60, help="steps of each measurement pass")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="batched workload: skip the serial plain comparison")

@@ -87,3 +87,24 @@ def test_bench_refuses_multi_gpu():
     r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
                        text=True, timeout=120)
     assert r.returncode == 2 and "one GPU" in r.stderr and r.stdout == ""
+
+
+def test_cpp_mirror_compiles_and_links(tmp_path):
+    # include/litiv_bgs.hpp (header-only IIBackgroundSubtractor mirror) against the built library
+    src = tmp_path / "use.cpp"
+    src.write_text('#include "litiv_bgs.hpp"\n'
+                   "int main(int argc, char**) {\n"
+                   "  if (argc > 5) {\n"
+                   "    litiv_amd::BackgroundSubtractorSuBSENSE s;\n"
+                   "    litiv_amd::Image<uint8_t> m; litiv_amd::ImageView v;\n"
+                   "    s.initialize(v); s.apply(v, m); s.refreshModel(0.1f);\n"
+                   "    litiv_amd::Image<uint16_t> d; s.getBackgroundDescriptorsImage(d);\n"
+                   "    litiv_amd::BackgroundSubtractorLOBSTER l; l.getBackgroundImage(m);\n"
+                   "  }\n"
+                   "  return 0;\n"
+                   "}\n")
+    exe = tmp_path / "use"
+    r = subprocess.run(["g++", "-std=c++17", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"), str(src),
+                        "-L", os.path.join(ROOT, "litiv_amd"), "-llitiv_bgs", "-o", str(exe)],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr[-2000:]
