@@ -21,7 +21,12 @@
  *   - OpenCV whole-image ops are restated by their closed forms (orc_post.c);
  *   - built with -ffp-contract=off (the reference build allows contraction).
  *
- * Parity pins (what checks each function) are listed in orc_pins.txt.
+ * Pins: tests/test_oracle_pins.py checks each primitive against an outside reference (Random123
+ * Philox vectors, the reference's LBSP golden data, the reference's sampling table, scipy's
+ * morphology/median/hole filling, closed forms); tests/test_oracle_behaviour.py checks the whole
+ * apply() on scenes with known answers.
+ * Parity unpinned: the fp32 rounding of OpenCV's addWeighted / accumulateWeighted (restated here as
+ * fixed unfused formulas), orc_resize_area for non-integer factors.
  */
 #ifndef LITIV_ORACLE_ORC_H
 #define LITIV_ORACLE_ORC_H
