@@ -151,6 +151,13 @@ def run_batch(args, preset, n_streams, ref_frames):
     torch.cuda.synchronize()
     log("workload subsense_batch64: %d streams, %.2f Mpix per step, %d warm-up + %d timed steps, %.1f GB of frames"
         % (n_streams, px_step / 1e6, args.warmup, steps, sum(f.numel() for f in frames) / 1e9))
+    if args.calib_copy_mb:
+        # a device copy of known size, for scaling hardware byte counters in profiling runs
+        a = torch.empty(args.calib_copy_mb << 20, dtype=torch.uint8, device=dev)
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
     f0 = [s.frame(0) for s in seqs]
     subs = [BackgroundSubtractorSuBSENSE(stream_id=i, device=0) for i in range(n_streams)]
 
@@ -274,6 +281,11 @@ def run_batch(args, preset, n_streams, ref_frames):
         "fixpoint": {"mean_iters": round(float(np.mean(iters)), 3),
                      "mean_worklist_per_stream": [round(float(x), 1) for x in (work / len(iters))[1:6]]},
     }
+    if args.serve_seconds > 0:
+        log("serving: %d stream slots, %.0f s warm-up + %.0f s measured" % (args.serve_streams, args.serve_warmup,
+                                                                            args.serve_seconds))
+        result["serving"] = run_serve(args)
+        log("serving: %s" % json.dumps(result["serving"]))
     if not args.no_cpu:
         # streams 16g .. 16g+threads-1 of each resolution class g, one per thread, first frames
         threads = max(1, min(args.cpu_threads, 16))
@@ -287,6 +299,145 @@ def run_batch(args, preset, n_streams, ref_frames):
         result["cpu_baseline"] = cpu_baseline("subsense", classes, 0.0, args.cpu_budget)
         result["cpu_baseline"]["ratio_gpu_over_cpu"] = round(mpix / result["cpu_baseline"]["value"], 1)
     return result
+
+
+# ------------------------------------------------------------------------------------------------
+# serving: streams arrive and leave, frames arrive as Poisson processes (wall clock), the server
+# batches whatever is waiting
+# ------------------------------------------------------------------------------------------------
+def run_serve(args, log_prefix="serve"):
+    """256 stream slots; resolutions from the SERVE generator preset ({320x240, 640x480, 1280x720,
+    1920x1080} RGB with p = .3/.3/.2/.2); frames arrive at 30 fps (Poisson) per stream; lifetimes are
+    exponential with a mean of 2000 frames and an ended stream is replaced by a new one; models are
+    admitted against a 250 GB budget (streams that do not fit wait). Latency = frame arrival ->
+    completion seen by the host. Frames come from a per-stream ring of pre-generated device frames."""
+    import heapq
+
+    import torch
+    from litiv_amd import Server, synth
+
+    n_slots, fps, mean_life, ring = args.serve_streams, 30.0, 2000.0, 16
+    rng = np.random.default_rng(0x5E12E)
+    srv = Server(hbm_budget=250 << 30, max_inflight=2, max_batch=n_slots, fix_px_per_block=8192,
+                 cont_px_per_block=2048)
+    dev = torch.device("cuda:0")
+    incarnation = [0]
+    slots = {}      # server stream id -> dict
+    waiting = []    # stream configs that did not fit the budget yet
+
+    def new_stream():
+        k = incarnation[0]
+        incarnation[0] += 1
+        cfg = synth.preset(synth.SERVE, k)
+        cfg.frames = max(cfg.frames, ring)
+        seq = synth.Sequence(cfg)
+        return {"k": k, "seq": seq, "w": cfg.width, "h": cfg.height,
+                "life": max(1, int(rng.exponential(mean_life)))}
+
+    def admit(st, now):
+        seq = st["seq"]
+        frames = torch.empty((ring, seq.frame_bytes), dtype=torch.uint8, device=dev)
+        seq.frames_device(0, ring, frames.data_ptr())
+        torch.cuda.synchronize()
+        sid = srv.open(frames.data_ptr(), st["w"], st["h"], 3, stream_id=st["k"])
+        if sid is None:
+            return None
+        st.update(sid=sid, frames=frames, masks=torch.empty((ring, st["w"] * st["h"]), dtype=torch.uint8, device=dev),
+                  t=0, done=0, closing=False, next=now + rng.exponential(1.0 / fps))
+        slots[sid] = st
+        return sid
+
+    t0 = time.perf_counter()
+    for _ in range(n_slots):
+        st = new_stream()
+        if admit(st, t0) is None:
+            waiting.append(st)
+    log("%s: %d streams admitted, %d waiting, %.1f GB of models, setup %.1f s" % (
+        log_prefix, len(slots), len(waiting), srv.stats()["bytes_used"] / 1e9, time.perf_counter() - t0))
+    start = time.perf_counter()
+    # the arrival clock starts now
+    for sid, st in slots.items():
+        st["next"] = start + rng.exponential(1.0 / fps)
+    heap = [(st["next"], sid) for sid, st in slots.items()]
+    heapq.heapify(heap)
+    t_warm, t_end = start + args.serve_warmup, start + args.serve_warmup + args.serve_seconds
+    arrival = {}
+    lat, px_done, n_done_window = [], 0, 0
+    npx = {}
+    replaced = 0
+    tag_seq = 0
+    last_log = start
+    while True:
+        now = time.perf_counter()
+        if now >= t_end:
+            break
+        while heap and heap[0][0] <= now:
+            ta, sid = heapq.heappop(heap)
+            st = slots.get(sid)
+            if st is None or st["closing"] or st.get("next") != ta:
+                continue
+            i = st["t"] % ring
+            tag_seq += 1
+            srv.submit(sid, st["frames"][i].data_ptr(), st["masks"][i].data_ptr(), 0.0, tag=tag_seq)
+            arrival[tag_seq] = ta
+            npx[tag_seq] = st["w"] * st["h"]
+            st["t"] += 1
+            if st["t"] >= st["life"]:
+                st["closing"] = True
+                srv.close(sid)
+            else:
+                st["next"] = ta + rng.exponential(1.0 / fps)
+                heapq.heappush(heap, (st["next"], sid))
+        srv.pump()
+        done = srv.poll()
+        if len(done):
+            tnow = time.perf_counter()
+            for tag in done["tag"]:
+                tag = int(tag)
+                ta = arrival.pop(tag)
+                p = npx.pop(tag)
+                if ta >= t_warm:
+                    lat.append(tnow - ta)
+                if t_warm <= tnow:
+                    px_done += p
+                    n_done_window += 1
+        # streams that ended and finished: replace them (the new stream may have to wait)
+        for sid in [s for s, st in slots.items() if st["closing"]]:
+            if srv.retired(sid):
+                del slots[sid]
+                waiting.append(new_stream())
+                replaced += 1
+        while waiting:
+            sid = admit(waiting[0], time.perf_counter())
+            if sid is None:
+                break
+            waiting.pop(0)
+            heapq.heappush(heap, (slots[sid]["next"], sid))
+        if now - last_log > 5.0:
+            last_log = now
+            sst = srv.stats()
+            log("%s: t=%.0fs open %d pending %d done %d steps %d" % (log_prefix, now - start, sst["open_streams"],
+                                                                   sst["pending_frames"], sst["frames_done"],
+                                                                   sst["steps"]))
+    window = t_end - t_warm
+    sst = srv.stats()
+    srv.drain()
+    srv.close_server()
+    lat_ms = np.array(lat) * 1e3
+    offered = sum(st["w"] * st["h"] for st in slots.values()) * fps / 1e6
+    return {
+        "streams": n_slots, "open_at_end": sst["open_streams"], "waiting_at_end": len(waiting),
+        "replaced": replaced, "window_s": round(window, 1),
+        "latency_ms": {"p50": round(float(np.percentile(lat_ms, 50)), 3) if len(lat_ms) else None,
+                       "p99": round(float(np.percentile(lat_ms, 99)), 3) if len(lat_ms) else None,
+                       "max": round(float(lat_ms.max()), 3) if len(lat_ms) else None,
+                       "frames": int(len(lat_ms))},
+        "mpix_s": round(px_done / window / 1e6, 1),
+        "offered_mpix_s": round(offered, 1),
+        "mean_batch": round(sst["frames_launched"] / max(1, sst["steps"]), 1),
+        "pending_at_end": sst["pending_frames"],
+        "model_gb": round(sst["bytes_used"] / 1e9, 1),
+    }
 
 
 # ------------------------------------------------------------------------------------------------
@@ -412,6 +563,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="batched workload: skip the serial plain comparison")
     ap.add_argument("--no-tune", action="store_true", help="batched workload: default launch options")
+    ap.add_argument("--calib-copy-mb", type=int, default=0, help="profiling runs: one device copy of this size first")
+    ap.add_argument("--serve-seconds", type=float, default=20.0,
+                    help="batched workload: measured seconds of the serving run (0 = skip)")
+    ap.add_argument("--serve-warmup", type=float, default=5.0)
+    ap.add_argument("--serve-streams", type=int, default=256)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; multi-GPU runs (--gpus %d) are not part of it. Streams "

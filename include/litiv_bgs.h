@@ -25,6 +25,7 @@ extern "C" {
 #define LVBGS_ERR_HIP (-3)      /* HIP runtime error */
 #define LVBGS_ERR_ALLOC (-4)    /* device memory allocation failed */
 #define LVBGS_ERR_DEVICE (-5)   /* device-side consistency check failed */
+#define LVBGS_ERR_BUDGET (-6)   /* a server's device-memory budget cannot hold the model */
 
 enum { LVBGS_SUBSENSE = 1, LVBGS_LOBSTER = 2 };
 
@@ -58,6 +59,14 @@ void lvbgs_destroy(lvbgs_ctx* ctx);
  * of the same size, else the full frame); otherwise a 0/255 mask of the same size. */
 int lvbgs_initialize(lvbgs_ctx* ctx, const uint8_t* img, int width, int height, int channels, size_t stride,
                      const uint8_t* roi, size_t roi_stride);
+
+/* initialize from a device-resident, tightly packed frame (same contract otherwise). */
+int lvbgs_initialize_device(lvbgs_ctx* ctx, const void* d_img, int width, int height, int channels,
+                            const uint8_t* roi, size_t roi_stride);
+
+/* Device bytes a context of this geometry allocates at initialize (model + per-step buffers);
+ * 0 for invalid arguments. params may be NULL (defaults). */
+size_t lvbgs_model_bytes(int algo, const lvbgs_params* params, int width, int height, int channels);
 
 /* IIBackgroundSubtractor::apply(image, fgmask, learningRate) (BackgroundSubtractionUtils.hpp:35;
  * SuBSENSE .cpp:188-612, LOBSTER .cpp:459-581). Host pointers, synchronous. fgmask receives the
@@ -167,6 +176,7 @@ typedef struct lvbgs_batch_options {
     int fixpoint_launches;  /* full-grid fix-point iterations before the one-workgroup tail */
     int fixpoint_blocks;    /* fix-point grid of a stream, at most */
     int fix_px_per_block;   /*   ... and ceil(pixels / this) when > 0 */
+    int fetch_depth;        /* sample loads in flight per pixel in the sequential match (2 or 4) */
 } lvbgs_batch_options;
 int lvbgs_batch_default_options(lvbgs_batch_options* out);
 int lvbgs_batch_get_options(const lvbgs_batch* b, lvbgs_batch_options* out);
@@ -179,6 +189,52 @@ int lvbgs_batch_synchronize(lvbgs_batch* b);
  * HIP events on the batch stream, summed by lvbgs_batch_take_timing (which synchronizes) */
 int lvbgs_batch_set_timing(lvbgs_batch* b, const char* stage);
 int lvbgs_batch_take_timing(lvbgs_batch* b, double* total_ms, int* count);
+
+/* ---- serving (no counterpart in the reference) ----
+ * A server keeps SuBSENSE models resident on one device and batches whatever frames are waiting:
+ * each step takes the oldest waiting frame of every stream that has one, and frames submitted
+ * while steps run join the next step (continuous batching). Streams are admitted against a
+ * device-memory budget; closed streams' contexts are kept for reuse by streams of the same
+ * geometry. A stream's results equal those of the same stream run alone. Frames and masks are
+ * device pointers that must stay valid until their completion is polled. Not thread-safe. */
+typedef struct lvbgs_server lvbgs_server;
+typedef struct lvbgs_server_options {
+    size_t hbm_budget;            /* bytes of stream contexts (lvbgs_model_bytes) held at most */
+    int max_inflight;             /* steps queued on the device at once (1..16) */
+    int max_batch;                /* streams per step at most */
+    lvbgs_batch_options launch;   /* launch options of each step (use_graph is ignored) */
+} lvbgs_server_options;
+typedef struct lvbgs_done {
+    uint64_t tag;                 /* the submit call's tag */
+    int32_t stream;
+    int32_t status;               /* LVBGS_OK */
+} lvbgs_done;
+typedef struct lvbgs_server_stats {
+    uint64_t steps, frames_submitted, frames_launched, frames_done;
+    uint64_t bytes_used, pending_frames;
+    uint32_t open_streams, pooled_contexts, queued_steps, reserved;
+} lvbgs_server_stats;
+int lvbgs_server_default_options(lvbgs_server_options* out);
+int lvbgs_server_create(int device, const lvbgs_server_options* opts, lvbgs_server** out);
+void lvbgs_server_destroy(lvbgs_server* s);
+const char* lvbgs_server_last_error(const lvbgs_server* s);
+void* lvbgs_server_stream(lvbgs_server* s);
+/* admits a SuBSENSE stream initialized from d_img0 (blocking); LVBGS_ERR_BUDGET when it does not fit */
+int lvbgs_server_open(lvbgs_server* s, const lvbgs_params* params, const void* d_img0, int width, int height,
+                      int channels, int* stream_out);
+/* no further frames; the stream's waiting frames still complete, then its id may be reused */
+int lvbgs_server_close(lvbgs_server* s, int stream);
+int lvbgs_server_submit(lvbgs_server* s, int stream, const void* d_img, void* d_mask, double lr, uint64_t tag);
+/* non-blocking: collects finished steps, launches steps while the device queue has room;
+ * returns the number of steps launched or < 0 */
+int lvbgs_server_pump(lvbgs_server* s);
+/* up to cap completions since the last call, in completion order; returns the count */
+int lvbgs_server_poll(lvbgs_server* s, lvbgs_done* out, int cap);
+/* blocks until every submitted frame has completed */
+int lvbgs_server_drain(lvbgs_server* s);
+int lvbgs_server_get_stats(lvbgs_server* s, lvbgs_server_stats* out);
+/* the context of an open stream (getters and state reads after lvbgs_server_drain) */
+lvbgs_ctx* lvbgs_server_context(lvbgs_server* s, int stream);
 
 #ifdef __cplusplus
 }

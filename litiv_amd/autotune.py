@@ -1,7 +1,8 @@
 """Launch-option autotuner for batches of SuBSENSE streams.
 
 The batch's launch options (grid scaling of the fix-point and cooperative match phases per stream,
-depth of the sequential match phase, number of full-grid fix-point launches) change the order of
+depth of the sequential match phase, sample loads in flight per lane, number of full-grid fix-point
+launches) change the order of
 evaluation but never the results, so the tuner can run its trials on live frames: it steps the
 batch through its warm-up frames, cycling over the candidate option sets (one step each, in turn,
 so content drift between frames affects every candidate alike), times each step's GPU work with
@@ -12,12 +13,12 @@ import numpy as np
 # fix_px_per_block / cont_px_per_block scale each stream's grid with its pixel count, so streams of
 # every resolution in the batch get a grid sized to their worklist
 CANDIDATES = [
-    {"fix_px_per_block": 8192, "fixpoint_launches": 3, "seq_limit": 255},
-    {"fix_px_per_block": 32768, "fixpoint_launches": 3, "seq_limit": 255},
-    {"fix_px_per_block": 2048, "fixpoint_launches": 3, "seq_limit": 255},
-    {"fix_px_per_block": 8192, "fixpoint_launches": 2, "seq_limit": 255},
-    {"fix_px_per_block": 8192, "fixpoint_launches": 4, "seq_limit": 255},
-    {"fix_px_per_block": 8192, "fixpoint_launches": 3, "seq_limit": 12, "cont_px_per_block": 2048},
+    {"fix_px_per_block": 8192, "fixpoint_launches": 3, "seq_limit": 255, "fetch_depth": 4},
+    {"fix_px_per_block": 8192, "fixpoint_launches": 3, "seq_limit": 255, "fetch_depth": 2},
+    {"fix_px_per_block": 32768, "fixpoint_launches": 3, "seq_limit": 255, "fetch_depth": 4},
+    {"fix_px_per_block": 8192, "fixpoint_launches": 2, "seq_limit": 255, "fetch_depth": 4},
+    {"fix_px_per_block": 8192, "fixpoint_launches": 4, "seq_limit": 255, "fetch_depth": 4},
+    {"fix_px_per_block": 8192, "fixpoint_launches": 3, "seq_limit": 12, "cont_px_per_block": 2048, "fetch_depth": 4},
 ]
 
 

@@ -59,8 +59,23 @@ class Stats(C.Structure):
 class BatchOptions(C.Structure):
     _fields_ = [("use_graph", C.c_int), ("seq_limit", C.c_int), ("cont_blocks", C.c_int),
                 ("cont_px_per_block", C.c_int), ("fixpoint_launches", C.c_int), ("fixpoint_blocks", C.c_int),
-                ("fix_px_per_block", C.c_int)]
+                ("fix_px_per_block", C.c_int), ("fetch_depth", C.c_int)]
 
+
+class ServerOptions(C.Structure):
+    _fields_ = [("hbm_budget", C.c_size_t), ("max_inflight", C.c_int), ("max_batch", C.c_int),
+                ("launch", BatchOptions)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("steps", C.c_uint64), ("frames_submitted", C.c_uint64), ("frames_launched", C.c_uint64),
+                ("frames_done", C.c_uint64), ("bytes_used", C.c_uint64), ("pending_frames", C.c_uint64),
+                ("open_streams", C.c_uint32), ("pooled_contexts", C.c_uint32), ("queued_steps", C.c_uint32),
+                ("reserved", C.c_uint32)]
+
+
+DONE_DTYPE = np.dtype([("tag", np.uint64), ("stream", np.int32), ("status", np.int32)])
+ERR_BUDGET = -6
 
 _lib = None
 
@@ -108,6 +123,25 @@ def lib():
         L.lvbgs_set_fixpoint_schedule.argtypes = [vp, i, i]
         L.lvbgs_set_match_schedule.argtypes = [vp, i, i]
         L.lvbgs_debug_rng.argtypes = [vp, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, vp]
+        L.lvbgs_initialize_device.argtypes = [vp, vp, i, i, i, vp, sz]
+        L.lvbgs_model_bytes.argtypes = [i, C.POINTER(Params), i, i, i]
+        L.lvbgs_model_bytes.restype = sz
+        L.lvbgs_server_default_options.argtypes = [C.POINTER(ServerOptions)]
+        L.lvbgs_server_create.argtypes = [i, C.POINTER(ServerOptions), C.POINTER(vp)]
+        L.lvbgs_server_destroy.argtypes = [vp]
+        L.lvbgs_server_last_error.argtypes = [vp]
+        L.lvbgs_server_last_error.restype = C.c_char_p
+        L.lvbgs_server_stream.argtypes = [vp]
+        L.lvbgs_server_stream.restype = vp
+        L.lvbgs_server_open.argtypes = [vp, C.POINTER(Params), vp, i, i, i, C.POINTER(i)]
+        L.lvbgs_server_close.argtypes = [vp, i]
+        L.lvbgs_server_submit.argtypes = [vp, i, vp, vp, C.c_double, C.c_uint64]
+        L.lvbgs_server_pump.argtypes = [vp]
+        L.lvbgs_server_poll.argtypes = [vp, vp, i]
+        L.lvbgs_server_drain.argtypes = [vp]
+        L.lvbgs_server_get_stats.argtypes = [vp, C.POINTER(ServerStats)]
+        L.lvbgs_server_context.argtypes = [vp, i]
+        L.lvbgs_server_context.restype = vp
         L.lvbgs_batch_create.argtypes = [C.POINTER(vp), i, C.POINTER(vp)]
         L.lvbgs_batch_destroy.argtypes = [vp]
         L.lvbgs_batch_last_error.argtypes = [vp]
@@ -166,9 +200,20 @@ class BackgroundSubtractor:
         self._h = h
         self.shape = None
 
+    @classmethod
+    def _borrow(cls, handle, params, shape):
+        """A non-owning view of a context owned elsewhere (a server's stream)."""
+        self = cls.__new__(cls)
+        self.params = params
+        self._h = C.c_void_p(handle)
+        self._owned = False
+        self.shape = shape
+        return self
+
     def close(self):
         if getattr(self, "_h", None):
-            lib().lvbgs_destroy(self._h)
+            if getattr(self, "_owned", True):
+                lib().lvbgs_destroy(self._h)
             self._h = None
 
     def __del__(self):
@@ -178,6 +223,11 @@ class BackgroundSubtractor:
         if rc < 0:
             raise LvbgsError(rc, lib().lvbgs_last_error(self._h).decode())
         return rc
+
+    def initialize_device(self, d_img, width, height, channels):
+        """initialize() from a device-resident, tightly packed frame."""
+        self._check(lib().lvbgs_initialize_device(self._h, C.c_void_p(d_img), width, height, channels, None, 0))
+        self.shape = (height, width, channels)
 
     # ---- reference interface ----
     def initialize(self, img, roi=None):
@@ -395,3 +445,102 @@ class Batch:
         ms, n = C.c_double(), C.c_int()
         self._check(lib().lvbgs_batch_take_timing(self._h, C.byref(ms), C.byref(n)))
         return ms.value, n.value
+
+
+class Server:
+    """Serving loop over resident SuBSENSE models (continuous batching, device-memory admission).
+    Frames and masks are device pointers; completions come back through poll()."""
+
+    def __init__(self, device=0, hbm_budget=250 << 30, max_inflight=2, max_batch=256, **launch):
+        o = ServerOptions()
+        lib().lvbgs_server_default_options(C.byref(o))
+        o.hbm_budget = int(hbm_budget)
+        o.max_inflight = int(max_inflight)
+        o.max_batch = int(max_batch)
+        for k, v in launch.items():
+            if not hasattr(o.launch, k):
+                raise TypeError("unknown launch option %r" % k)
+            setattr(o.launch, k, int(v))
+        h = C.c_void_p()
+        rc = lib().lvbgs_server_create(int(device), C.byref(o), C.byref(h))
+        if rc != OK:
+            raise LvbgsError(rc, "lvbgs_server_create failed")
+        self._h = h
+        self._params = {}
+        self._shape = {}
+        self._done = np.zeros(4096, DONE_DTYPE)
+
+    def close_server(self):
+        if getattr(self, "_h", None):
+            lib().lvbgs_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close_server()
+
+    def _check(self, rc):
+        if rc < 0:
+            raise LvbgsError(rc, lib().lvbgs_server_last_error(self._h).decode())
+        return rc
+
+    @staticmethod
+    def model_bytes(width, height, channels, **params):
+        p = default_params(SUBSENSE)
+        for k, v in params.items():
+            setattr(p, k, v)
+        return lib().lvbgs_model_bytes(SUBSENSE, C.byref(p), width, height, channels)
+
+    def open(self, d_img0, width, height, channels, **params):
+        """Admits a stream; returns its id, or None when its model does not fit the budget."""
+        p = default_params(SUBSENSE)
+        for k, v in params.items():
+            if not hasattr(p, k):
+                raise TypeError("unknown parameter %r" % k)
+            setattr(p, k, v)
+        sid = C.c_int()
+        rc = lib().lvbgs_server_open(self._h, C.byref(p), C.c_void_p(d_img0), width, height, channels, C.byref(sid))
+        if rc == ERR_BUDGET:
+            return None
+        self._check(rc)
+        self._params[sid.value] = p
+        self._shape[sid.value] = (height, width, channels)
+        return sid.value
+
+    def close(self, sid):
+        self._check(lib().lvbgs_server_close(self._h, int(sid)))
+
+    def submit(self, sid, d_img, d_mask=None, lr=0.0, tag=0):
+        self._check(lib().lvbgs_server_submit(self._h, int(sid), C.c_void_p(d_img), C.c_void_p(d_mask or 0),
+                                              float(lr), int(tag)))
+
+    def pump(self):
+        return self._check(lib().lvbgs_server_pump(self._h))
+
+    def poll(self):
+        """Completions since the last call: structured array (tag, stream, status)."""
+        out = []
+        while True:
+            n = self._check(lib().lvbgs_server_poll(self._h, self._done.ctypes.data, len(self._done)))
+            out.append(self._done[:n].copy())
+            if n < len(self._done):
+                break
+        return np.concatenate(out)
+
+    def drain(self):
+        self._check(lib().lvbgs_server_drain(self._h))
+
+    def stats(self):
+        st = ServerStats()
+        self._check(lib().lvbgs_server_get_stats(self._h, C.byref(st)))
+        return {k: getattr(st, k) for k, _ in ServerStats._fields_}
+
+    def retired(self, sid):
+        """True once a closed stream's last frame has completed (its id may then be reused)."""
+        return not lib().lvbgs_server_context(self._h, int(sid))
+
+    def member(self, sid):
+        """The open stream's subtractor (state reads and getters after drain()); not owned."""
+        h = lib().lvbgs_server_context(self._h, int(sid))
+        if not h:
+            raise LvbgsError(ERR_ARG, "unknown stream")
+        return BackgroundSubtractorSuBSENSE._borrow(h, self._params[sid], self._shape[sid])

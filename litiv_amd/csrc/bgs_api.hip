@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "../../include/litiv_bgs.h"
+#include "bgs_ctx.hpp"
 #include "bgs_kernels.hpp"
 
 using namespace lvbgs;
@@ -39,32 +40,6 @@ uint8_t sat_u8_host(float v) {
 }
 
 }  // namespace
-
-struct lvbgs_ctx {
-    int algo = 0;
-    lvbgs_params p{};
-    int device = 0;
-    hipStream_t stream = nullptr;
-    std::string err;
-    // geometry / state
-    bool initialized = false;
-    int W = 0, H = 0, C = 0;
-    size_t npx = 0, n_roi = 0, orig_roi = 0;
-    std::vector<uint8_t> roi;  // validated ROI (host copy)
-    int roi_w = 0, roi_h = 0;
-    // device
-    uint8_t* arena = nullptr;
-    size_t arena_size = 0;
-    View v{};
-    uint8_t* d_frame = nullptr;  // staging for host-pointer apply / initialize
-    uint8_t* d_mask = nullptr;
-    StepOptions opt{};
-    uint32_t apply_count = 0;
-    // timing
-    int timed_stage = 0;
-    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pairs;
-    size_t ev_used = 0;
-};
 
 static int set_err(lvbgs_ctx* c, int code, const char* msg) {
     if (c) c->err = msg;
@@ -160,42 +135,61 @@ extern "C" int lvbgs_validate_roi(int w, int h, uint8_t* roi, size_t stride) {
     return LVBGS_OK;
 }
 
-// allocates (or re-uses) the device arena for the current geometry and carves the View
-static int allocate(lvbgs_ctx* c) {
+// carves the arena starting at a.base into the View (a.base == nullptr: sizing dry run)
+static void carve(lvbgs_ctx* c, Arena& ar) {
     const size_t npx = c->npx, C = (size_t)c->C, N = c->p.bg_samples;
     const bool sub = c->algo == LVBGS_SUBSENSE;
     const int dsW = c->W / 8, dsH = c->H / 8;
     const size_t nds = (size_t)(dsW > 0 ? dsW : 1) * (size_t)(dsH > 0 ? dsH : 1) * C;
-    // dry run to size the arena
-    Arena a;
     View& v = c->v;
-    auto carve = [&](Arena& ar) {
-        v.roi = ar.take<uint8_t>(npx);
-        v.last_color = ar.take<uint8_t>(npx * C);
-        v.last_desc = ar.take<uint16_t>(npx * C);
-        v.sA = ar.take<uint8_t>(N * npx * (C == 3 ? 8 : 2));
-        v.sB = ar.take<uint8_t>(N * npx);
-        float** maps[] = {&v.T, &v.R, &v.V, &v.MLD, &v.DminLT, &v.DminST, &v.RawLT, &v.RawST, &v.FinLT, &v.FinST};
-        for (auto m : maps) *m = sub ? ar.take<float>(npx) : nullptr;
-        uint8_t** u8s[] = {&v.unstable, &v.blinks, &v.last_raw, &v.last_raw_blink, &v.last_fg, &v.last_dil_inv,
-                           &v.raw, &v.pre, &v.mh, &v.dec};
-        for (auto m : u8s) *m = ar.take<uint8_t>(npx);
-        v.label = ar.take<uint32_t>(npx);
-        v.desc_cur = ar.take<uint16_t>(npx * C);
-        v.rec = ar.take<uint32_t>(npx);
-        v.mres = ar.take<uint32_t>(npx);
-        v.wl0 = ar.take<uint32_t>(npx);
-        v.wl1 = ar.take<uint32_t>(npx);
-        v.cont = ar.take<uint32_t>(npx);
-        v.stamp = ar.take<unsigned long long>(npx);
-        v.dsLT = ar.take<float>(nds);
-        v.dsST = ar.take<float>(nds);
-        v.sc = ar.take<Scalars>(1);
-        c->d_frame = ar.take<uint8_t>(npx * C);
-        c->d_mask = ar.take<uint8_t>(npx);
-    };
-    a.base = nullptr;
-    carve(a);
+    v.roi = ar.take<uint8_t>(npx);
+    v.last_color = ar.take<uint8_t>(npx * C);
+    v.last_desc = ar.take<uint16_t>(npx * C);
+    v.sA = ar.take<uint8_t>(N * npx * (C == 3 ? 12 : 4));
+    float** maps[] = {&v.T, &v.R, &v.V, &v.MLD, &v.DminLT, &v.DminST, &v.RawLT, &v.RawST, &v.FinLT, &v.FinST};
+    for (auto m : maps) *m = sub ? ar.take<float>(npx) : nullptr;
+    uint8_t** u8s[] = {&v.unstable, &v.blinks, &v.last_raw, &v.last_raw_blink, &v.last_fg, &v.last_dil_inv,
+                       &v.raw, &v.pre, &v.mh, &v.dec};
+    for (auto m : u8s) *m = ar.take<uint8_t>(npx);
+    v.label = ar.take<uint32_t>(npx);
+    v.desc_cur = ar.take<uint16_t>(npx * C);
+    v.rec = ar.take<uint32_t>(npx);
+    v.mres = ar.take<uint32_t>(npx);
+    v.wl0 = ar.take<uint32_t>(npx);
+    v.wl1 = ar.take<uint32_t>(npx);
+    v.cont = ar.take<uint32_t>(npx);
+    v.stamp = ar.take<unsigned long long>(npx);
+    v.dsLT = ar.take<float>(nds);
+    v.dsST = ar.take<float>(nds);
+    v.sc = ar.take<Scalars>(1);
+    c->d_frame = ar.take<uint8_t>(npx * C);
+    c->d_mask = ar.take<uint8_t>(npx);
+}
+
+size_t lvbgs_arena_bytes(int algo, const lvbgs_params& p, int width, int height, int channels) {
+    lvbgs_ctx c;
+    c.algo = algo;
+    c.p = p;
+    c.W = width;
+    c.H = height;
+    c.C = channels;
+    c.npx = (size_t)width * height;
+    Arena a;
+    carve(&c, a);
+    return a.off + 256;
+}
+
+extern "C" size_t lvbgs_model_bytes(int algo, const lvbgs_params* p, int width, int height, int channels) {
+    lvbgs_params d;
+    if (lvbgs_default_params(algo, &d) != LVBGS_OK || width <= 0 || height <= 0 || (channels != 1 && channels != 3))
+        return 0;
+    return lvbgs_arena_bytes(algo, p ? *p : d, width, height, channels);
+}
+
+// allocates (or re-uses) the device arena for the current geometry and carves the View
+static int allocate(lvbgs_ctx* c) {
+    Arena a;
+    carve(c, a);
     const size_t need = a.off + 256;
     if (need != c->arena_size) {
         release_device(c);
@@ -207,7 +201,7 @@ static int allocate(lvbgs_ctx* c) {
     }
     Arena b;
     b.base = c->arena;
-    carve(b);
+    carve(c, b);
     return hipMemsetAsync(c->arena, 0, c->arena_size, c->stream) == hipSuccess ? LVBGS_OK
                                                                                : set_err(c, LVBGS_ERR_HIP, "memset");
 }
@@ -222,8 +216,22 @@ static int download_scalars(lvbgs_ctx* c, Scalars& s) {
     return hip_check(c, hipStreamSynchronize(c->stream), "sync");
 }
 
+static int initialize(lvbgs_ctx* c, const uint8_t* img, bool device_img, int w, int h, int ch, size_t stride,
+                      const uint8_t* roi, size_t roi_stride);
+
 extern "C" int lvbgs_initialize(lvbgs_ctx* c, const uint8_t* img, int w, int h, int ch, size_t stride,
                                 const uint8_t* roi, size_t roi_stride) {
+    return initialize(c, img, false, w, h, ch, stride, roi, roi_stride);
+}
+
+extern "C" int lvbgs_initialize_device(lvbgs_ctx* c, const void* d_img, int w, int h, int ch, const uint8_t* roi,
+                                       size_t roi_stride) {
+    return initialize(c, (const uint8_t*)d_img, true, w, h, ch, (size_t)w * (size_t)(ch > 0 ? ch : 1), roi,
+                      roi_stride);
+}
+
+static int initialize(lvbgs_ctx* c, const uint8_t* img, bool device_img, int w, int h, int ch, size_t stride,
+                      const uint8_t* roi, size_t roi_stride) {
     if (!c) return LVBGS_ERR_ARG;
     c->err.clear();
     if (!img || w <= 0 || h <= 0) return set_err(c, LVBGS_ERR_ARG, "provided image for initialization must be non-empty");
@@ -339,16 +347,20 @@ extern "C" int lvbgs_initialize(lvbgs_ctx* c, const uint8_t* img, int w, int h, 
     s.upper_cap = upper;
     if ((rc = upload_scalars(c, s))) return rc;
     if ((rc = hip_check(c, hipMemcpyAsync((void*)v.roi, nr.data(), npx, hipMemcpyHostToDevice, c->stream), "roi"))) return rc;
-    if ((rc = hip_check(c, hipMemcpy2DAsync(c->d_frame, (size_t)w * ch, img, stride, (size_t)w * ch, (size_t)h,
-                                            hipMemcpyHostToDevice, c->stream), "frame")))
-        return rc;
+    const uint8_t* d_init = img;
+    if (!device_img) {
+        if ((rc = hip_check(c, hipMemcpy2DAsync(c->d_frame, (size_t)w * ch, img, stride, (size_t)w * ch, (size_t)h,
+                                                hipMemcpyHostToDevice, c->stream), "frame")))
+            return rc;
+        d_init = c->d_frame;
+    }
     if (c->algo == LVBGS_SUBSENSE) {
         const unsigned b = (unsigned)((npx + 255) / 256);
         hipLaunchKernelGGL(k_fill_f32, dim3(b), dim3(256), 0, c->stream, v.T, npx, lower);
         hipLaunchKernelGGL(k_fill_f32, dim3(b), dim3(256), 0, c->stream, v.R, npx, 1.0f);
         hipLaunchKernelGGL(k_fill_f32, dim3(b), dim3(256), 0, c->stream, v.V, npx, 10.0f);
     }
-    launch_init_desc(v, c->d_frame, c->stream);
+    launch_init_desc(v, d_init, c->stream);
     // refreshModel(1.0f) for SuBSENSE (.cpp:184), refreshModel(1.0f, true) for LOBSTER (.cpp:455)
     launch_refresh(v, 1.0f, c->algo == LVBGS_LOBSTER ? 1 : 0, 0u, 0u, c->stream);
     s.refresh_seq = 1;
@@ -584,31 +596,30 @@ extern "C" long long lvbgs_get_state(lvbgs_ctx* c, const char* name, void* out, 
     std::vector<uint8_t> unpack;
     if (n == "samples_color" || n == "samples_desc") {
         // unpack the sample planes to the oracle layout: N x (H x W x C)
-        std::vector<uint8_t> a(N * npx * (C == 3 ? 8 : 2)), b(N * npx);
-        if (hipMemcpy(a.data(), v.sA, a.size(), hipMemcpyDeviceToHost) != hipSuccess ||
-            hipMemcpy(b.data(), v.sB, b.size(), hipMemcpyDeviceToHost) != hipSuccess)
-            return LVBGS_ERR_HIP;
+        const size_t rec = C == 3 ? 12 : 4;
+        std::vector<uint8_t> a(N * npx * rec);
+        if (hipMemcpy(a.data(), v.sA, a.size(), hipMemcpyDeviceToHost) != hipSuccess) return LVBGS_ERR_HIP;
         const bool col = n == "samples_color";
         bytes = N * npx * C * (col ? 1 : 2);
         if (bytes > cap) return LVBGS_ERR_ARG;
         for (size_t i = 0; i < N * npx; ++i) {
+            uint32_t w[3] = {0, 0, 0};
+            std::memcpy(w, a.data() + i * rec, rec);
             if (C == 3) {
-                uint64_t w;
-                std::memcpy(&w, a.data() + i * 8, 8);
                 if (col) {
                     uint8_t* o = (uint8_t*)out + i * 3;
-                    o[0] = (uint8_t)(w >> 48);
-                    o[1] = (uint8_t)(w >> 56);
-                    o[2] = b[i];
+                    o[0] = (uint8_t)(w[1] >> 16);
+                    o[1] = (uint8_t)(w[1] >> 24);
+                    o[2] = (uint8_t)w[2];
                 } else {
                     uint16_t* o = (uint16_t*)out + i * 3;
-                    o[0] = (uint16_t)w;
-                    o[1] = (uint16_t)(w >> 16);
-                    o[2] = (uint16_t)(w >> 32);
+                    o[0] = (uint16_t)w[0];
+                    o[1] = (uint16_t)(w[0] >> 16);
+                    o[2] = (uint16_t)w[1];
                 }
             } else {
-                if (col) ((uint8_t*)out)[i] = b[i];
-                else std::memcpy((uint16_t*)out + i, a.data() + i * 2, 2);
+                if (col) ((uint8_t*)out)[i] = (uint8_t)(w[0] >> 16);
+                else ((uint16_t*)out)[i] = (uint16_t)w[0];
             }
         }
         return (long long)bytes;
@@ -791,6 +802,7 @@ extern "C" int lvbgs_batch_create(lvbgs_ctx* const* ctxs, int n, lvbgs_batch** o
         b->opt.fixpoint_launches = d.fixpoint_launches;
         b->opt.fixpoint_blocks = d.fixpoint_blocks;
         b->opt.fix_px_per_block = d.fix_px_per_block;
+        b->opt.fetch_depth = d.fetch_depth;
     }
     hipSetDevice(b->device);
     int rc = LVBGS_OK;
@@ -833,6 +845,7 @@ extern "C" int lvbgs_batch_default_options(lvbgs_batch_options* o) {
     o->fixpoint_launches = 3;
     o->fixpoint_blocks = 256;
     o->fix_px_per_block = 8192;
+    o->fetch_depth = 4;
     return LVBGS_OK;
 }
 
@@ -845,6 +858,7 @@ extern "C" int lvbgs_batch_get_options(const lvbgs_batch* b, lvbgs_batch_options
     o->fixpoint_launches = b->opt.fixpoint_launches;
     o->fixpoint_blocks = b->opt.fixpoint_blocks;
     o->fix_px_per_block = b->opt.fix_px_per_block;
+    o->fetch_depth = b->opt.fetch_depth;
     return LVBGS_OK;
 }
 
@@ -852,7 +866,7 @@ extern "C" int lvbgs_batch_set_options(lvbgs_batch* b, const lvbgs_batch_options
     if (!b || !o) return LVBGS_ERR_ARG;
     if (o->seq_limit < 1 || o->seq_limit > 255 || o->cont_blocks < 1 || o->cont_blocks > 65535 ||
         o->fixpoint_launches < 0 || o->fixpoint_launches > 64 || o->fixpoint_blocks < 1 || o->fixpoint_blocks > 65535 ||
-        o->cont_px_per_block < 0 || o->fix_px_per_block < 0)
+        o->cont_px_per_block < 0 || o->fix_px_per_block < 0 || (o->fetch_depth != 2 && o->fetch_depth != 4))
         return bset_err(b, LVBGS_ERR_ARG, "invalid batch options");
     hipSetDevice(b->device);
     hipStreamSynchronize(b->stream);
@@ -863,6 +877,7 @@ extern "C" int lvbgs_batch_set_options(lvbgs_batch* b, const lvbgs_batch_options
     b->opt.fixpoint_launches = o->fixpoint_launches;
     b->opt.fixpoint_blocks = o->fixpoint_blocks;
     b->opt.fix_px_per_block = o->fix_px_per_block;
+    b->opt.fetch_depth = o->fetch_depth;
     return batch_plan(b);
 }
 

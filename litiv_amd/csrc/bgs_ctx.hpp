@@ -1,0 +1,40 @@
+// bgs_ctx.hpp -- the per-stream context behind lvbgs_ctx (shared by the C ABI and the server).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../include/litiv_bgs.h"
+#include "bgs_kernels.hpp"
+
+struct lvbgs_ctx {
+    int algo = 0;
+    lvbgs_params p{};
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    // geometry / state
+    bool initialized = false;
+    int W = 0, H = 0, C = 0;
+    size_t npx = 0, n_roi = 0, orig_roi = 0;
+    std::vector<uint8_t> roi;  // validated ROI (host copy)
+    int roi_w = 0, roi_h = 0;
+    // device
+    uint8_t* arena = nullptr;
+    size_t arena_size = 0;
+    lvbgs::View v{};
+    uint8_t* d_frame = nullptr;  // staging for host-pointer apply / initialize
+    uint8_t* d_mask = nullptr;
+    lvbgs::StepOptions opt{};
+    uint32_t apply_count = 0;
+    // timing
+    int timed_stage = 0;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pairs;
+    size_t ev_used = 0;
+};
+
+
+// device bytes of the model and per-step buffers of a stream of this geometry (one arena)
+size_t lvbgs_arena_bytes(int algo, const lvbgs_params& p, int width, int height, int channels);

@@ -73,35 +73,68 @@ LV_D uint32_t cur_frame(const View& v) { return v.sc->frame_idx + 1u; }
 // ----------------------------------------------------------------------------------------------
 // sample planes
 // ----------------------------------------------------------------------------------------------
+// one sample record: 12 bytes (RGB) or 4 bytes (grey), a single load or store per sample
+struct alignas(4) Rec12 {
+    uint32_t w0, w1, w2;
+};
+
+// raw sample in registers: ra = record words 0-1 (RGB) or word 0 (grey), rb = word 2 (RGB)
 template <int C>
-LV_D void load_sample(const View& v, uint32_t s, uint32_t p, uint32_t col[C], uint32_t desc[C]) {
+LV_D void fetch_raw(const View& v, uint32_t s, uint32_t p, uint64_t& ra, uint32_t& rb) {
     const size_t idx = (size_t)s * v.npx + p;
     if constexpr (C == 3) {
-        const uint64_t a = ((const uint64_t*)v.sA)[idx];
-        desc[0] = (uint32_t)(a & 0xFFFFu);
-        desc[1] = (uint32_t)((a >> 16) & 0xFFFFu);
-        desc[2] = (uint32_t)((a >> 32) & 0xFFFFu);
-        col[0] = (uint32_t)((a >> 48) & 0xFFu);
-        col[1] = (uint32_t)(a >> 56);
-        col[2] = v.sB[idx];
+        const Rec12 r = ((const Rec12*)v.sA)[idx];
+        ra = (uint64_t)r.w0 | ((uint64_t)r.w1 << 32);
+        rb = r.w2;
     } else {
-        desc[0] = ((const uint16_t*)v.sA)[idx];
-        col[0] = v.sB[idx];
+        ra = ((const uint32_t*)v.sA)[idx];
+        rb = 0;
     }
+}
+
+template <int C>
+LV_D void decode_raw(uint64_t ra, uint32_t rb, uint32_t col[C], uint32_t desc[C]) {
+    if constexpr (C == 3) {
+        desc[0] = (uint32_t)(ra & 0xFFFFu);
+        desc[1] = (uint32_t)((ra >> 16) & 0xFFFFu);
+        desc[2] = (uint32_t)((ra >> 32) & 0xFFFFu);
+        col[0] = (uint32_t)((ra >> 48) & 0xFFu);
+        col[1] = (uint32_t)(ra >> 56);
+        col[2] = rb & 0xFFu;
+    } else {
+        desc[0] = (uint32_t)(ra & 0xFFFFu);
+        col[0] = (uint32_t)((ra >> 16) & 0xFFu);
+    }
+}
+
+template <int C>
+LV_D void encode_raw(const uint32_t col[C], const uint32_t desc[C], uint64_t& ra, uint32_t& rb) {
+    if constexpr (C == 3) {
+        ra = (uint64_t)desc[0] | ((uint64_t)desc[1] << 16) | ((uint64_t)desc[2] << 32) | ((uint64_t)col[0] << 48) |
+             ((uint64_t)col[1] << 56);
+        rb = col[2];
+    } else {
+        ra = (uint64_t)desc[0] | ((uint64_t)col[0] << 16);
+        rb = 0;
+    }
+}
+
+template <int C>
+LV_D void load_sample(const View& v, uint32_t s, uint32_t p, uint32_t col[C], uint32_t desc[C]) {
+    uint64_t ra;
+    uint32_t rb;
+    fetch_raw<C>(v, s, p, ra, rb);
+    decode_raw<C>(ra, rb, col, desc);
 }
 
 template <int C>
 LV_D void store_sample(const View& v, uint32_t s, uint32_t p, const uint32_t col[C], const uint32_t desc[C]) {
     const size_t idx = (size_t)s * v.npx + p;
-    if constexpr (C == 3) {
-        const uint64_t a = (uint64_t)desc[0] | ((uint64_t)desc[1] << 16) | ((uint64_t)desc[2] << 32) |
-                           ((uint64_t)col[0] << 48) | ((uint64_t)col[1] << 56);
-        ((uint64_t*)v.sA)[idx] = a;
-        v.sB[idx] = (uint8_t)col[2];
-    } else {
-        ((uint16_t*)v.sA)[idx] = (uint16_t)desc[0];
-        v.sB[idx] = (uint8_t)col[0];
-    }
+    uint64_t ra;
+    uint32_t rb;
+    encode_raw<C>(col, desc, ra, rb);
+    if constexpr (C == 3) ((Rec12*)v.sA)[idx] = Rec12{(uint32_t)ra, (uint32_t)(ra >> 32), rb};
+    else ((uint32_t*)v.sA)[idx] = (uint32_t)ra;
 }
 
 // current colour and intra descriptor of pixel q (the value it writes into samples)
@@ -307,10 +340,10 @@ LV_D uint32_t candidate_writers(const View& v, int x, int y) {
 // Ordered scan s = 0..N-1 until REQ matches (SUB:366-395, LOB:480-495). Samples overwritten by
 // applied writers (mask `applied`, slot bitmap `sbm`) are taken from the writer; the last writer
 // in raster order wins. Two evaluation orders give the same result as the sequential loop:
-//   scan_seq  : one lane per pixel, samples in order, kFetch loads in flight, up to `limit`;
+//   scan_seq  : one lane per pixel, samples in order, KF loads in flight, up to `limit`, with
+//               at most kRepl overwritten slots substituted from the thread's LDS entries;
 //   scan_coop : kGroup lanes per pixel evaluate kGroup consecutive samples at once; a ballot finds
 //               the REQ-th match in sample order, distances are reduced only up to it.
-constexpr int kFetch = 2;
 constexpr int kGroup = 8;
 constexpr uint32_t kBig = 0xFFFFu;
 
@@ -333,29 +366,11 @@ LV_D uint64_t applied_slots(const View& v, int x, int y, uint32_t applied) {
     return sbm;
 }
 
-template <int C>
-LV_D void fetch_raw(const View& v, uint32_t s, uint32_t p, uint64_t& ra, uint32_t& rb) {
-    const size_t idx = (size_t)s * v.npx + p;
-    if constexpr (C == 3) ra = ((const uint64_t*)v.sA)[idx];
-    else ra = ((const uint16_t*)v.sA)[idx];
-    rb = v.sB[idx];
-}
-
 // decodes a fetched sample, replacing it by the last applied writer's value when one wrote slot s
 template <int C>
 LV_D void sample_value(const View& v, const uint8_t* frame, int x, int y, uint32_t s, uint32_t applied, uint64_t sbm,
                        uint64_t ra, uint32_t rb, uint32_t col[C], uint32_t desc[C]) {
-    if constexpr (C == 3) {
-        desc[0] = (uint32_t)(ra & 0xFFFFu);
-        desc[1] = (uint32_t)((ra >> 16) & 0xFFFFu);
-        desc[2] = (uint32_t)((ra >> 32) & 0xFFFFu);
-        col[0] = (uint32_t)((ra >> 48) & 0xFFu);
-        col[1] = (uint32_t)(ra >> 56);
-        col[2] = rb;
-    } else {
-        desc[0] = (uint32_t)ra;
-        col[0] = rb;
-    }
+    decode_raw<C>(ra, rb, col, desc);
     if ((sbm >> (s & 63u)) & 1u) {
         for (int i = 11; i >= 0; --i) {
             if (!((applied >> i) & 1u)) continue;
@@ -368,23 +383,88 @@ LV_D void sample_value(const View& v, const uint8_t* frame, int x, int y, uint32
     }
 }
 
-template <int C, int ALGO, int RS>
-LV_D void scan_seq(const View& v, const uint8_t* frame, const uint8_t* lut, const PxCtx<C>& px, int x, int y,
-                   uint32_t applied, uint64_t sbm, ScanState& st, uint32_t limit) {
+// Overwritten slots of a pixel with their replacement values (the last applied writer's current
+// value), kept per thread in LDS so the sequential scan's loop carries only a slot bitmap: a sample
+// whose slot bit is set looks its slot up in the thread's entries. At most kRepl distinct slots;
+// pixels with more (very rare) go to the cooperative phase.
+constexpr int kRepl = 4;
+struct ReplLds {
+    uint32_t slot[kRepl][TPB];
+    uint32_t w0[kRepl][TPB], w1[kRepl][TPB], w2[kRepl][TPB];
+};
+
+// one LDS table per kernel, whichever channel counts the kernel instantiates
+LV_D ReplLds& repl_lds() {
+    __shared__ ReplLds r;
+    return r;
+}
+
+template <int C>
+LV_D void raw_current(const View& v, const uint8_t* frame, uint32_t q, uint64_t& ra, uint32_t& rb) {
+    uint32_t col[C], desc[C];
+    current_value<C>(v, frame, q, col, desc);
+    encode_raw<C>(col, desc, ra, rb);
+}
+
+// Fills the thread's entries from the applied writers, the latest in raster order first (it owns
+// its slot). Returns the slot bitmap (mod 64) and the entry count, or false beyond kRepl slots.
+template <int C>
+LV_D bool build_repl(const View& v, const uint8_t* frame, int x, int y, uint32_t applied, uint64_t& sbm, int& n) {
+    ReplLds& R = repl_lds();
+    const int t = (int)threadIdx.x;
+    sbm = 0;
+    n = 0;
+    for (uint32_t m = applied; m;) {
+        const int i = 31 - __builtin_clz(m);
+        m &= ~(1u << i);
+        const uint32_t q = (uint32_t)((y + kPredDY[i]) * v.W + (x + kPredDX[i]));
+        const uint32_t sl = rec_slot_diff(v.rec[q]);
+        bool seen = false;
+        for (int k = 0; k < n; ++k) seen = seen || R.slot[k][t] == sl;
+        if (seen) continue;
+        if (n == kRepl) return false;
+        uint64_t ra;
+        uint32_t rb;
+        raw_current<C>(v, frame, q, ra, rb);
+        R.slot[n][t] = sl;
+        R.w0[n][t] = (uint32_t)ra;
+        R.w1[n][t] = (uint32_t)(ra >> 32);
+        R.w2[n][t] = rb;
+        sbm |= 1ull << (sl & 63u);
+        ++n;
+    }
+    return true;
+}
+
+template <int C, int ALGO, int RS, int KF>
+LV_D void scan_seq(const View& v, const uint8_t* lut, const PxCtx<C>& px, int x, int y, uint64_t sbm, int nrep,
+                   ScanState& st, uint32_t limit) {
     const uint32_t p = (uint32_t)(y * v.W + x);
     const uint32_t REQ = (uint32_t)v.REQ;
+    const ReplLds& R = repl_lds();
+    const int t = (int)threadIdx.x;
     while (st.s < limit && st.good < REQ) {
-        uint64_t ra[kFetch];
-        uint32_t rb[kFetch];
+        uint64_t ra[KF];
+        uint32_t rb[KF];
         const uint32_t s0 = st.s;
 #pragma unroll
-        for (int j = 0; j < kFetch; ++j)
+        for (int j = 0; j < KF; ++j)
             if (s0 + j < limit) fetch_raw<C>(v, s0 + j, p, ra[j], rb[j]);
 #pragma unroll
-        for (int j = 0; j < kFetch; ++j) {
+        for (int j = 0; j < KF; ++j) {
             if (st.s >= limit || st.good >= REQ) break;
+            uint64_t a = ra[j];
+            uint32_t b = rb[j];
+            if ((sbm >> (st.s & 63u)) & 1u) {
+                for (int k = 0; k < nrep; ++k)
+                    if (R.slot[k][t] == st.s) {
+                        a = (uint64_t)R.w0[k][t] | ((uint64_t)R.w1[k][t] << 32);
+                        b = R.w2[k][t];
+                        break;
+                    }
+            }
             uint32_t col[C], desc[C];
-            sample_value<C>(v, frame, x, y, st.s, applied, sbm, ra[j], rb[j], col, desc);
+            decode_raw<C>(a, b, col, desc);
             uint32_t td = 0, ts = 0;
             if (test_sample<C, ALGO, RS>(px, lut, col, desc, td, ts)) {
                 st.md = td < st.md ? td : st.md;
@@ -531,10 +611,15 @@ LV_D void describe_body(const View& v, uint32_t bid, uint32_t nblk) {
             }
             nz = pc >= (C == 1 ? 2u : 4u);
             const uint32_t frame_no = cur_frame(v);
-            uint32_t d[4], d4[4];
+            uint32_t d[4];
             draw4(v.seed_lo, v.seed_hi, v.stream_id, frame_no, PH_APPLY, 0, p, 0, d);
-            draw4(v.seed_lo, v.seed_hi, v.stream_id, frame_no, PH_APPLY, 0, p, 1, d4);
             const uint32_t N = (uint32_t)v.N;
+            // draw 4 (group 1) is the diffusion slot: drawn only when a diffusion gate is open
+            auto diff_slot = [&]() {
+                uint32_t d4[4];
+                draw4(v.seed_lo, v.seed_hi, v.stream_id, frame_no, PH_APPLY, 0, p, 1, d4);
+                return d4[0] % N;
+            };
             uint32_t r;
             if constexpr (ALGO == kSuBSENSE) {
                 const float Rv = v.R[p];
@@ -556,9 +641,10 @@ LV_D void describe_body(const View& v, uint32_t bid, uint32_t nblk) {
                     ghost_fwd = v.RawST[t] > 0.995f && v.MLD[t] < 0.010f;
                 }
                 const uint32_t tg = (uint32_t)((ty - y + 2) * 5 + (tx - x + 2));
+                const uint32_t sd = (gate_a || gate_b) ? diff_slot() : 0u;
                 r = tg | ((uint32_t)gate_a << 5) | ((uint32_t)gate_b << 6) | ((uint32_t)ghost_fwd << 7) |
                     ((uint32_t)self_bg << 8) | ((uint32_t)self_fg << 9) | (unst_new << 10) | ((d[1] % N) << 16) |
-                    ((d4[0] % N) << 24);
+                    (sd << 24);
             } else {
                 const uint64_t L = isinf(lr) ? ~0ull : (uint64_t)ceil(lr);
                 const bool self_bg = mod64(d[0], L) == 0u;
@@ -566,7 +652,8 @@ LV_D void describe_body(const View& v, uint32_t bid, uint32_t nblk) {
                 int tx, ty;
                 neighbor_pos(d[3], true, x, y, v.W, v.H, tx, ty);
                 const uint32_t tg = (uint32_t)((ty - y + 2) * 5 + (tx - x + 2));
-                r = tg | ((uint32_t)gate_a << 5) | ((uint32_t)self_bg << 8) | ((d[1] % N) << 16) | ((d4[0] % N) << 24);
+                const uint32_t sd = gate_a ? diff_slot() : 0u;
+                r = tg | ((uint32_t)gate_a << 5) | ((uint32_t)self_bg << 8) | ((d[1] % N) << 16) | (sd << 24);
             }
             v.rec[p] = r;
         }
@@ -612,7 +699,7 @@ LV_D void stage_patch(const uint8_t* frame, int W, int x, int y, uint8_t* patch,
 // speculative match: raster-earlier writers assumed to decide as in the previous frame
 // ----------------------------------------------------------------------------------------------
 struct PredTile;
-template <int C, int ALGO>
+template <int C, int ALGO, int KF>
 LV_D bool match_pixel(const View& v, const uint8_t* frame, const uint8_t* lut, const FrameTile<C>& tile,
                       const PredTile& pred, int lx, int ly, int x, int y, uint32_t p, uint32_t seq_limit,
                       uint32_t& examined);
@@ -648,7 +735,7 @@ LV_D uint32_t applied_writers_tile(const PredTile& t, int lx, int ly) {
     return m;
 }
 
-template <int C, int ALGO>
+template <int C, int ALGO, int KF>
 LV_D void match_body(const View& v, uint32_t bid, uint32_t nblk) {
     const uint8_t* frame = v.frame;
     const uint32_t seq_limit = v.seq_limit;
@@ -667,7 +754,7 @@ LV_D void match_body(const View& v, uint32_t bid, uint32_t nblk) {
     uint32_t examined = 0;
     bool cont = false;
     if (x < v.W && y < v.H && v.roi[p])
-        cont = !match_pixel<C, ALGO>(v, frame, lut, tile, pred, lx, ly, x, y, p, seq_limit, examined);
+        cont = !match_pixel<C, ALGO, KF>(v, frame, lut, tile, pred, lx, ly, x, y, p, seq_limit, examined);
     // unfinished pixels go to the cooperative phase (one atomic per wavefront)
     const unsigned long long bal = __ballot(cont);
     if (bal) {
@@ -682,7 +769,7 @@ LV_D void match_body(const View& v, uint32_t bid, uint32_t nblk) {
 
 // Phase A: sequential scan of at most v_seq_limit samples. Returns true when the pixel is final;
 // otherwise its partial state is left in mres for the cooperative phase.
-template <int C, int ALGO>
+template <int C, int ALGO, int KF>
 LV_D bool match_pixel(const View& v, const uint8_t* frame, const uint8_t* lut, const FrameTile<C>& tile,
                       const PredTile& pred, int lx, int ly, int x, int y, uint32_t p, uint32_t seq_limit,
                       uint32_t& examined) {
@@ -690,10 +777,13 @@ LV_D bool match_pixel(const View& v, const uint8_t* frame, const uint8_t* lut, c
     build_ctx<C, ALGO, FrameTile<C>::LW * C>(v, frame, &tile.px[(ly * FrameTile<C>::LW + lx) * C], x, y, px);
     // guess: raster-earlier writers decide as in the previous frame
     const uint32_t applied = applied_writers_tile(pred, lx, ly);
-    const uint64_t sbm = applied_slots(v, x, y, applied);
+    uint64_t sbm;
+    int nrep;
     ScanState st = scan_init<C>();
     const uint32_t N = (uint32_t)v.N;
-    scan_seq<C, ALGO, FrameTile<C>::LW * C>(v, frame, lut, px, x, y, applied, sbm, st, seq_limit < N ? seq_limit : N);
+    // more than kRepl overwritten slots (very rare): the whole scan goes to the cooperative phase
+    const uint32_t limit = build_repl<C>(v, frame, x, y, applied, sbm, nrep) ? (seq_limit < N ? seq_limit : N) : 0u;
+    scan_seq<C, ALGO, FrameTile<C>::LW * C, KF>(v, lut, px, x, y, sbm, nrep, st, limit);
     examined = st.s;
     v.mres[p] = pack_state(st);
     if (st.good < (uint32_t)v.REQ && st.s < N) return false;
@@ -727,10 +817,11 @@ LV_D void match_cont_body(const View& v, uint32_t bid, uint32_t nblk) {
     const uint8_t* frame = v.frame;
     __shared__ uint8_t lut[256];
     __shared__ uint8_t patches[TPB / kGroup][25 * C];
+    const uint32_t n = v.sc->cont_count;
+    if (n == 0) return;  // uniform: nothing was left to this phase
     load_lut(lut, v.sc);
     __syncthreads();
     uint8_t* patch = patches[threadIdx.x / kGroup];
-    const uint32_t n = v.sc->cont_count;
     const uint32_t ngroups = nblk * (TPB / kGroup);
     const uint32_t sub = threadIdx.x & (kGroup - 1);
     const uint32_t g0 = (bid * TPB + threadIdx.x) / kGroup;
@@ -1004,10 +1095,11 @@ LV_D void update_body(const View& v, uint32_t bid, uint32_t nblk) {
     for (int i = threadIdx.x; i < LW * LH; i += TPB) {
         const int ly = i / LW, lx = i % LW, gx = x0 + lx - 2, gy = y0 + ly - 2;
         const bool in = gx >= 0 && gx < v.W && gy >= 0 && gy < v.H;
-        const size_t q = (size_t)gy * v.W + gx;
-        const uint32_t r = in ? v.rec[q] : kRecNone;
+        const size_t q = in ? (size_t)gy * v.W + gx : 0;
+        const uint32_t r = in ? v.rec[q] : kRecNone;  // both loads issued together
+        const uint8_t d = v.dec[q];
         s_rec[ly][lx] = r;
-        s_bg[ly][lx] = (rec_tgt(r) != kRecNone && v.dec[q] == 0) ? 1 : 0;
+        s_bg[ly][lx] = (rec_tgt(r) != kRecNone && d == 0) ? 1 : 0;
     }
     __syncthreads();
     const int lx = threadIdx.x % TW, ly = threadIdx.x / TW, x = x0 + lx, y = y0 + ly;
@@ -1017,39 +1109,42 @@ LV_D void update_body(const View& v, uint32_t bid, uint32_t nblk) {
         // ghost test of writers at or after p reads p's updated maps (SUB:438-441)
         bool ghost_new = false;
         if constexpr (ALGO == kSuBSENSE) ghost_new = v.RawST[p] > 0.995f && v.MLD[p] < 0.010f;
-        // Writers are visited in reverse raster order, so the first write seen for a slot is the
-        // last one the sequential loop performs; later-seen writes to a written slot are skipped.
-        uint64_t done0 = 0, done1 = 0, done2 = 0, done3 = 0;  // slots already final (N <= 255)
-#pragma unroll 1
-        for (int j = 24; j >= 0; --j) {
+        // Writes landing on p, as bits in sequential order: writer j of the 5x5 window (raster
+        // order) owns bit 2j (its self update, j == 12 only) and bit 2j+1 (its diffusion into p).
+        uint64_t wm = 0;
+#pragma unroll
+        for (int j = 0; j < 25; ++j) {
             const int dy = j / 5 - 2, dx = j % 5 - 2;
             const uint32_t r = s_rec[ly + 2 + dy][lx + 2 + dx];
-            const uint32_t tg = rec_tgt(r);
-            if (tg == kRecNone) continue;
             const bool bg = s_bg[ly + 2 + dy][lx + 2 + dx] != 0;
-            // up to two writes by this writer, in sequential order: self update (j == 12), then diffusion
-            uint32_t wslot[2];
-            int nwr = 0;
-            if (j == 12 && (bg ? rec_self_bg(r) : (ALGO == kSuBSENSE && rec_self_fg(r)))) wslot[nwr++] = rec_slot_self(r);
-            if (bg && tg == (uint32_t)((2 - dy) * 5 + (2 - dx)) &&
+            if (j == 12 && rec_tgt(r) != kRecNone && (bg ? rec_self_bg(r) : (ALGO == kSuBSENSE && rec_self_fg(r))))
+                wm |= 1ull << (2 * j);
+            if (bg && rec_tgt(r) == (uint32_t)((2 - dy) * 5 + (2 - dx)) &&
                 (j < 12 ? fwd_gate(r) : (rec_gate_a(r) || (rec_gate_b(r) && ghost_new))))
-                wslot[nwr++] = rec_slot_diff(r);
-            for (int k = nwr - 1; k >= 0; --k) {
-                const uint32_t sl = wslot[k];
-                const uint64_t bit = 1ull << (sl & 63u);
-                const uint32_t word = sl >> 6;
-                const uint64_t dw = word == 0 ? done0 : word == 1 ? done1 : word == 2 ? done2 : done3;
-                ++nw;
-                if (dw & bit) continue;
-                if (word == 0) done0 |= bit;
-                else if (word == 1) done1 |= bit;
-                else if (word == 2) done2 |= bit;
-                else done3 |= bit;
-                const uint32_t q = (uint32_t)((y + dy) * v.W + (x + dx));
-                uint32_t col[C], desc[C];
-                current_value<C>(v, frame, q, col, desc);
-                store_sample<C>(v, sl, p, col, desc);
-            }
+                wm |= 1ull << (2 * j + 1);
+        }
+        // Visited from the last write back, so the first write seen for a slot is the one the
+        // sequential loop leaves in place; earlier writes to a written slot are skipped.
+        uint64_t done0 = 0, done1 = 0, done2 = 0, done3 = 0;  // slots already final (N <= 255)
+        nw = (uint32_t)__popcll(wm);
+        while (wm) {
+            const int b = 63 - __clzll((long long)wm);
+            wm &= ~(1ull << b);
+            const int j = b >> 1, dy = j / 5 - 2, dx = j % 5 - 2;
+            const uint32_t r = s_rec[ly + 2 + dy][lx + 2 + dx];
+            const uint32_t sl = (b & 1) ? rec_slot_diff(r) : rec_slot_self(r);
+            const uint64_t bit = 1ull << (sl & 63u);
+            const uint32_t word = sl >> 6;
+            const uint64_t dw = word == 0 ? done0 : word == 1 ? done1 : word == 2 ? done2 : done3;
+            if (dw & bit) continue;
+            if (word == 0) done0 |= bit;
+            else if (word == 1) done1 |= bit;
+            else if (word == 2) done2 |= bit;
+            else done3 |= bit;
+            const uint32_t q = (uint32_t)((y + dy) * v.W + (x + dx));
+            uint32_t col[C], desc[C];
+            current_value<C>(v, frame, q, col, desc);
+            store_sample<C>(v, sl, p, col, desc);
         }
     }
     block_add<TPB>(nw, v.sc->wr_part);
@@ -1110,12 +1205,21 @@ LV_D void ccl_local_body(const View& v, uint32_t bid, uint32_t nblk) {
     __shared__ uint32_t lab[CT * CT];
     const int ntx = (v.W + CT - 1) / CT;
     const int x0 = ((int)bid % ntx) * CT, y0 = ((int)bid / ntx) * CT;
+    int any_fg = 0;
     for (int i = threadIdx.x; i < CT * CT; i += TPB) {
         const int gx = x0 + i % CT, gy = y0 + i / CT;
         const bool bgpx = gx < v.W && gy < v.H && v.pre[(size_t)gy * v.W + gx] == 0;
         lab[i] = bgpx ? (uint32_t)i : 0xFFFFFFFFu;
+        any_fg |= bgpx ? 0 : 1;
     }
-    __syncthreads();
+    // a tile inside the image with no foreground is one 4-connected component rooted at its
+    // first pixel (the minimum index, as the union below would find)
+    if (!__syncthreads_or(any_fg)) {
+        const uint32_t root = (uint32_t)(y0 * v.W + x0);
+        for (int i = threadIdx.x; i < CT * CT; i += TPB)
+            v.label[(size_t)(y0 + i / CT) * v.W + (x0 + i % CT)] = root;
+        return;
+    }
     for (int i = threadIdx.x; i < CT * CT; i += TPB) {
         if (lab[i] == 0xFFFFFFFFu) continue;
         const int lx = i % CT, ly = i / CT;
@@ -1471,6 +1575,11 @@ LV_D void lobster_end_body(const View& v, uint32_t bid, uint32_t nblk) {
 // refreshModel (SUB:80-105; LOB:410-440). pending = 1: auto-reset request from k_decide (also
 // T := 1 everywhere, SUB:592); otherwise an explicit call with the given parameters.
 template <int C, int ALGO>
+LV_D void refresh_pixel(const View& v, uint32_t p, int use_pending, int force_fg, uint32_t frame, uint32_t ev,
+                        uint32_t nref, uint32_t start);
+
+// A grid-stride sweep, so the per-step launch (normally nothing pending) stays small.
+template <int C, int ALGO>
 LV_D void refresh_body(const View& v, uint32_t bid, uint32_t nblk, int use_pending, float frac, int force_fg, uint32_t frame, uint32_t ev) {
     if (use_pending) {
         if (!v.sc->refresh_pending) return;
@@ -1479,11 +1588,6 @@ LV_D void refresh_body(const View& v, uint32_t bid, uint32_t nblk, int use_pendi
         frame = v.sc->refresh_frame;
         ev = v.sc->refresh_ev;
     }
-    const uint32_t p = bid * TPB + threadIdx.x;
-    if (p >= v.npx) return;
-    if (use_pending && ALGO == kSuBSENSE) v.T[p] = 1.0f;
-    if (!v.roi[p]) return;
-    if (!(force_fg || !v.last_fg[p])) return;
     const uint32_t N = (uint32_t)v.N;
     const uint32_t nref = frac < 1.0f ? (uint32_t)(frac * (float)N) : N;
     uint32_t start = 0;
@@ -1492,6 +1596,17 @@ LV_D void refresh_body(const View& v, uint32_t bid, uint32_t nblk, int use_pendi
         draw4(v.seed_lo, v.seed_hi, v.stream_id, frame, PH_REFRESH, ev, 0xFFFFFFFFu, 0, d);
         start = d[0] % N;
     }
+    for (uint32_t p = bid * TPB + threadIdx.x; p < v.npx; p += nblk * TPB)
+        refresh_pixel<C, ALGO>(v, p, use_pending, force_fg, frame, ev, nref, start);
+}
+
+template <int C, int ALGO>
+LV_D void refresh_pixel(const View& v, uint32_t p, int use_pending, int force_fg, uint32_t frame, uint32_t ev,
+                        uint32_t nref, uint32_t start) {
+    if (use_pending && ALGO == kSuBSENSE) v.T[p] = 1.0f;
+    if (!v.roi[p]) return;
+    if (!(force_fg || !v.last_fg[p])) return;
+    const uint32_t N = (uint32_t)v.N;
     const int x = (int)(p % (uint32_t)v.W), y = (int)(p / (uint32_t)v.W);
     uint32_t d[4];
     for (uint32_t j = 0; j < nref; ++j) {
@@ -1630,8 +1745,8 @@ LV_D void for_batch_block(const BatchArgs& b, F&& f) {
 
 template <int C, int ALGO>
 __global__ void __launch_bounds__(TPB) k_describe(View v) { describe_body<C, ALGO>(v, LV_SINGLE_BID); }
-template <int C, int ALGO>
-__global__ void __launch_bounds__(TPB) k_match(View v) { match_body<C, ALGO>(v, LV_SINGLE_BID); }
+template <int C, int ALGO, int KF>
+__global__ void __launch_bounds__(TPB) k_match(View v) { match_body<C, ALGO, KF>(v, LV_SINGLE_BID); }
 template <int C, int ALGO>
 __global__ void __launch_bounds__(TPB) k_match_cont(View v) { match_cont_body<C, ALGO>(v, LV_SINGLE_BID); }
 template <int C, int ALGO>
@@ -1672,7 +1787,8 @@ __global__ void __launch_bounds__(TPB) k_refresh(View v, int use_pending, float 
         else body<1 __VA_OPT__(, ) __VA_ARGS__>(v, bid, nblk);                                \
     })
 __global__ void __launch_bounds__(TPB) kb_describe(BatchArgs b) { LV_BATCH_C(describe_body, kSuBSENSE); }
-__global__ void __launch_bounds__(TPB) kb_match(BatchArgs b) { LV_BATCH_C(match_body, kSuBSENSE); }
+template <int KF>
+__global__ void __launch_bounds__(TPB) kb_match(BatchArgs b) { LV_BATCH_C(match_body, kSuBSENSE, KF); }
 __global__ void __launch_bounds__(TPB) kb_match_cont(BatchArgs b) { LV_BATCH_C(match_cont_body, kSuBSENSE); }
 __global__ void __launch_bounds__(TPB) kb_fixpoint(BatchArgs b, uint32_t k) {
     for_batch_block(b, [&](const View& v, uint32_t bid, uint32_t nblk) {
@@ -1719,6 +1835,7 @@ void kernel_grids(const View& v, const StepOptions& opt, uint32_t g[kNumGrids]) 
     g[kGridOne] = 1u;
     g[kGridFix] = (uint32_t)opt.fixpoint_blocks;
     if (opt.fix_px_per_block > 0) g[kGridFix] = std::min(g[kGridFix], cdiv(v.npx, (size_t)opt.fix_px_per_block));
+    g[kGridSweep] = std::min(cdiv(v.npx, TPB), 128u);
     g[kGridCont] = (uint32_t)opt.cont_blocks;
     if (opt.cont_px_per_block > 0) g[kGridCont] = std::min(g[kGridCont], cdiv(v.npx, (size_t)opt.cont_px_per_block));
 }
@@ -1730,8 +1847,11 @@ static void frame_step_t(const View& v, const StepOptions& opt, hipStream_t st) 
     if (opt.timed_stage == 2) (void)hipEventRecord(opt.ev_begin, st);
     hipLaunchKernelGGL((k_describe<C, ALGO>), dim3(g[kGridTile]), dim3(TPB), 0, st, v);
     if (opt.timed_stage == 1) (void)hipEventRecord(opt.ev_begin, st);
-    hipLaunchKernelGGL((k_match<C, ALGO>), dim3(g[kGridTile]), dim3(TPB), 0, st, v);
-    if (opt.seq_limit < v.N) hipLaunchKernelGGL((k_match_cont<C, ALGO>), dim3(g[kGridCont]), dim3(TPB), 0, st, v);
+    if (opt.fetch_depth == 4) hipLaunchKernelGGL((k_match<C, ALGO, 4>), dim3(g[kGridTile]), dim3(TPB), 0, st, v);
+    else hipLaunchKernelGGL((k_match<C, ALGO, 2>), dim3(g[kGridTile]), dim3(TPB), 0, st, v);
+    // the cooperative phase finishes what the sequential phase left (scans cut at seq_limit and
+    // pixels with more than two overwritten slots); it exits at once when nothing was left
+    hipLaunchKernelGGL((k_match_cont<C, ALGO>), dim3(g[kGridCont]), dim3(TPB), 0, st, v);
     if (opt.timed_stage == 1) (void)hipEventRecord(opt.ev_end, st);
     for (uint32_t k = 1; k <= (uint32_t)opt.fixpoint_launches; ++k)
         hipLaunchKernelGGL((k_fixpoint<C, ALGO>), dim3(g[kGridFix]), dim3(TPB), 0, st, v, k);
@@ -1748,7 +1868,7 @@ static void frame_step_t(const View& v, const StepOptions& opt, hipStream_t st) 
         hipLaunchKernelGGL(k_dilate_final, dim3(g[kGridPost]), dim3(TPB), 0, st, v);
         if (g[kGridCell]) hipLaunchKernelGGL((k_downsample<C>), dim3(g[kGridCell]), dim3(TPB), 0, st, v);
         hipLaunchKernelGGL(k_decide, dim3(1), dim3(256), 0, st, v);
-        hipLaunchKernelGGL((k_refresh<C, ALGO>), dim3(g[kGridPix]), dim3(TPB), 0, st, v, 1, 0.f, 0, 0u, 0u);
+        hipLaunchKernelGGL((k_refresh<C, ALGO>), dim3(g[kGridSweep]), dim3(TPB), 0, st, v, 1, 0.f, 0, 0u, 0u);
     } else {
         hipLaunchKernelGGL(k_lobster_raw, dim3(g[kGridPix]), dim3(TPB), 0, st, v);
         hipLaunchKernelGGL((k_update<C, ALGO>), dim3(g[kGridTile]), dim3(TPB), 0, st, v);
@@ -1778,8 +1898,9 @@ void batch_step(const View* d_views, int ns, const uint32_t* const prefix[kNumGr
     if (opt.timed_stage == 2) (void)hipEventRecord(opt.ev_begin, st);
     hipLaunchKernelGGL(kb_describe, dim3(total[kGridTile]), dim3(TPB), 0, st, args(kGridTile));
     if (opt.timed_stage == 1) (void)hipEventRecord(opt.ev_begin, st);
-    hipLaunchKernelGGL(kb_match, dim3(total[kGridTile]), dim3(TPB), 0, st, args(kGridTile));
-    if (any_cont) hipLaunchKernelGGL(kb_match_cont, dim3(total[kGridCont]), dim3(TPB), 0, st, args(kGridCont));
+    if (opt.fetch_depth == 4) hipLaunchKernelGGL(kb_match<4>, dim3(total[kGridTile]), dim3(TPB), 0, st, args(kGridTile));
+    else hipLaunchKernelGGL(kb_match<2>, dim3(total[kGridTile]), dim3(TPB), 0, st, args(kGridTile));
+    hipLaunchKernelGGL(kb_match_cont, dim3(total[kGridCont]), dim3(TPB), 0, st, args(kGridCont));
     if (opt.timed_stage == 1) (void)hipEventRecord(opt.ev_end, st);
     for (uint32_t k = 1; k <= (uint32_t)opt.fixpoint_launches; ++k)
         hipLaunchKernelGGL(kb_fixpoint, dim3(total[kGridFix]), dim3(TPB), 0, st, args(kGridFix), k);
@@ -1795,7 +1916,7 @@ void batch_step(const View* d_views, int ns, const uint32_t* const prefix[kNumGr
     hipLaunchKernelGGL(kb_dilate_final, dim3(total[kGridPost]), dim3(TPB), 0, st, args(kGridPost));
     if (total[kGridCell]) hipLaunchKernelGGL(kb_downsample, dim3(total[kGridCell]), dim3(TPB), 0, st, args(kGridCell));
     hipLaunchKernelGGL(kb_decide, dim3(total[kGridOne]), dim3(256), 0, st, args(kGridOne));
-    hipLaunchKernelGGL(kb_refresh, dim3(total[kGridPix]), dim3(TPB), 0, st, args(kGridPix));
+    hipLaunchKernelGGL(kb_refresh, dim3(total[kGridSweep]), dim3(TPB), 0, st, args(kGridSweep));
     if (opt.timed_stage == 2) (void)hipEventRecord(opt.ev_end, st);
 }
 

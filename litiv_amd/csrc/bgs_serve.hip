@@ -1,0 +1,397 @@
+// bgs_serve.hip -- serving loop (include/litiv_bgs.h, lvbgs_server_*): SuBSENSE models resident on
+// one device, continuous batching of whatever frames are waiting, admission against a device-memory
+// budget. Host code only; the steps reuse the batched kernels of bgs_kernels.hip.
+//
+// Scheduling: every pump harvests finished steps (event queries) and, while fewer than
+// max_inflight steps are queued on the device, launches one more step made of the oldest waiting
+// frame of every stream that has one (round-robin start, at most max_batch streams). A stream may
+// have frames in several queued steps: the steps run in order on one HIP stream, so each frame
+// sees the model its predecessor left. Frames arriving while steps run join the next step.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <deque>
+#include <string>
+#include <vector>
+
+#include "../../include/litiv_bgs.h"
+#include "bgs_ctx.hpp"
+#include "bgs_kernels.hpp"
+
+using namespace lvbgs;
+
+namespace {
+
+struct Request {
+    const uint8_t* img;
+    uint8_t* mask;
+    double lr;
+    uint64_t tag;
+};
+
+struct Slot {
+    lvbgs_ctx* ctx = nullptr;
+    std::deque<Request> pending;
+    int inflight = 0;  // frames of this stream in queued steps
+    bool open = false;
+    bool closing = false;
+};
+
+struct QueuedStep {
+    hipEvent_t done;
+    std::vector<std::pair<int, uint64_t>> frames;  // (stream, tag)
+};
+
+}  // namespace
+
+struct lvbgs_server {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    lvbgs_server_options o{};
+    StepOptions opt{};
+    size_t used = 0;  // arena bytes of open and pooled contexts
+    std::vector<Slot> slots;
+    std::vector<int> free_ids;
+    std::vector<lvbgs_ctx*> pool;  // closed contexts whose arenas are kept for reuse
+    View* d_views = nullptr;
+    uint32_t* d_prefix = nullptr;
+    std::vector<uint8_t*> stage;  // pinned staging: max_batch Views + prefix tables
+    std::vector<hipEvent_t> stage_ev;
+    std::vector<bool> stage_live;
+    int stage_next = 0;
+    std::vector<hipEvent_t> ev_free;
+    std::deque<QueuedStep> queued;
+    std::vector<lvbgs_done> done;
+    int rr = 0;
+    lvbgs_server_stats st{};
+};
+
+static int srv_err(lvbgs_server* s, int code, const char* msg) {
+    if (s) s->err = msg;
+    return code;
+}
+
+static size_t stage_bytes(int max_batch) {
+    return sizeof(View) * (size_t)max_batch + sizeof(uint32_t) * (size_t)kNumGrids * (max_batch + 1);
+}
+
+extern "C" int lvbgs_server_default_options(lvbgs_server_options* o) {
+    if (!o) return LVBGS_ERR_ARG;
+    std::memset(o, 0, sizeof(*o));
+    o->hbm_budget = (size_t)250 << 30;
+    o->max_inflight = 2;
+    o->max_batch = 256;
+    lvbgs_batch_default_options(&o->launch);
+    o->launch.use_graph = 0;
+    return LVBGS_OK;
+}
+
+static void server_release(lvbgs_server* s) {
+    for (auto& sl : s->slots)
+        if (sl.ctx) lvbgs_destroy(sl.ctx);
+    for (auto* c : s->pool) lvbgs_destroy(c);
+    if (s->d_views) (void)hipFree(s->d_views);
+    if (s->d_prefix) (void)hipFree(s->d_prefix);
+    for (auto* p : s->stage) (void)hipHostFree(p);
+    for (auto e : s->stage_ev) (void)hipEventDestroy(e);
+    for (auto e : s->ev_free) (void)hipEventDestroy(e);
+    for (auto& q : s->queued) (void)hipEventDestroy(q.done);
+    if (s->stream) (void)hipStreamDestroy(s->stream);
+}
+
+extern "C" int lvbgs_server_create(int device, const lvbgs_server_options* opts, lvbgs_server** out) {
+    if (!out) return LVBGS_ERR_ARG;
+    *out = nullptr;
+    lvbgs_server_options o;
+    lvbgs_server_default_options(&o);
+    if (opts) o = *opts;
+    const lvbgs_batch_options& L = o.launch;
+    if (o.max_inflight < 1 || o.max_inflight > 16 || o.max_batch < 1 || o.max_batch > 4096 || L.seq_limit < 1 ||
+        L.seq_limit > 255 || L.cont_blocks < 1 || L.fixpoint_launches < 0 || L.fixpoint_launches > 64 ||
+        L.fixpoint_blocks < 1 || L.cont_px_per_block < 0 || L.fix_px_per_block < 0 ||
+        (L.fetch_depth != 2 && L.fetch_depth != 4))
+        return LVBGS_ERR_ARG;
+    if (hipSetDevice(device) != hipSuccess) return LVBGS_ERR_HIP;
+    lvbgs_server* s = new lvbgs_server();
+    s->device = device;
+    s->o = o;
+    s->opt.seq_limit = L.seq_limit;
+    s->opt.cont_blocks = L.cont_blocks;
+    s->opt.cont_px_per_block = L.cont_px_per_block;
+    s->opt.fixpoint_launches = L.fixpoint_launches;
+    s->opt.fixpoint_blocks = L.fixpoint_blocks;
+    s->opt.fix_px_per_block = L.fix_px_per_block;
+    s->opt.fetch_depth = L.fetch_depth;
+    bool ok = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) == hipSuccess &&
+              hipMalloc(&s->d_views, sizeof(View) * o.max_batch) == hipSuccess &&
+              hipMalloc(&s->d_prefix, sizeof(uint32_t) * kNumGrids * (o.max_batch + 1)) == hipSuccess;
+    const int nstage = o.max_inflight + 1;
+    for (int i = 0; ok && i < nstage; ++i) {
+        uint8_t* p = nullptr;
+        hipEvent_t e = nullptr;
+        ok = hipHostMalloc((void**)&p, stage_bytes(o.max_batch)) == hipSuccess &&
+             hipEventCreateWithFlags(&e, hipEventDisableTiming) == hipSuccess;
+        if (p) s->stage.push_back(p);
+        if (e) s->stage_ev.push_back(e);
+        s->stage_live.push_back(false);
+    }
+    for (int i = 0; ok && i < o.max_inflight; ++i) {
+        hipEvent_t e = nullptr;
+        ok = hipEventCreateWithFlags(&e, hipEventDisableTiming) == hipSuccess;
+        if (e) s->ev_free.push_back(e);
+    }
+    if (!ok) {
+        server_release(s);
+        delete s;
+        return LVBGS_ERR_HIP;
+    }
+    *out = s;
+    return LVBGS_OK;
+}
+
+extern "C" const char* lvbgs_server_last_error(const lvbgs_server* s) { return s ? s->err.c_str() : "null server"; }
+extern "C" void* lvbgs_server_stream(lvbgs_server* s) { return s ? (void*)s->stream : nullptr; }
+
+static bool same_model_params(const lvbgs_params& a, const lvbgs_params& b) {
+    return a.desc_dist_threshold == b.desc_dist_threshold && a.color_dist_threshold == b.color_dist_threshold &&
+           a.bg_samples == b.bg_samples && a.required_bg_samples == b.required_bg_samples &&
+           a.samples_for_mv_avgs == b.samples_for_mv_avgs && a.lbsp_threshold_offset == b.lbsp_threshold_offset &&
+           a.rel_lbsp_threshold == b.rel_lbsp_threshold && a.median_blur_kernel == b.median_blur_kernel;
+}
+
+// finished steps -> completions; closing streams whose work is done go to the pool
+static int harvest(lvbgs_server* s, bool wait_one) {
+    while (!s->queued.empty()) {
+        QueuedStep& q = s->queued.front();
+        const hipError_t e = wait_one ? hipEventSynchronize(q.done) : hipEventQuery(q.done);
+        wait_one = false;
+        if (e == hipErrorNotReady) break;
+        if (e != hipSuccess) return srv_err(s, LVBGS_ERR_HIP, "step failed");
+        for (auto& f : q.frames) {
+            s->done.push_back(lvbgs_done{f.second, f.first, LVBGS_OK});
+            Slot& sl = s->slots[(size_t)f.first];
+            --sl.inflight;
+        }
+        s->st.frames_done += q.frames.size();
+        s->ev_free.push_back(q.done);
+        s->queued.pop_front();
+    }
+    for (size_t i = 0; i < s->slots.size(); ++i) {
+        Slot& sl = s->slots[i];
+        if (sl.open && sl.closing && sl.pending.empty() && sl.inflight == 0) {
+            s->pool.push_back(sl.ctx);
+            sl = Slot();
+            s->free_ids.push_back((int)i);
+        }
+    }
+    return LVBGS_OK;
+}
+
+static int launch_step(lvbgs_server* s) {
+    const int nslots = (int)s->slots.size();
+    std::vector<int> members;
+    for (int k = 0; k < nslots && (int)members.size() < s->o.max_batch; ++k) {
+        const int i = (s->rr + k) % nslots;
+        if (s->slots[(size_t)i].open && !s->slots[(size_t)i].pending.empty()) members.push_back(i);
+    }
+    if (members.empty()) return LVBGS_OK;
+    s->rr = (members.back() + 1) % nslots;
+    const int n = (int)members.size();
+    const int si = s->stage_next;
+    s->stage_next = (si + 1) % (int)s->stage.size();
+    if (s->stage_live[(size_t)si] && hipEventSynchronize(s->stage_ev[(size_t)si]) != hipSuccess)
+        return srv_err(s, LVBGS_ERR_HIP, "staging");
+    View* hv = (View*)s->stage[(size_t)si];
+    uint32_t* hp = (uint32_t*)(s->stage[(size_t)si] + sizeof(View) * (size_t)s->o.max_batch);
+    QueuedStep q;
+    q.frames.reserve((size_t)n);
+    for (int k = 0; k < kNumGrids; ++k) hp[(size_t)k * (n + 1)] = 0;
+    for (int m = 0; m < n; ++m) {
+        Slot& sl = s->slots[(size_t)members[(size_t)m]];
+        const Request r = sl.pending.front();
+        sl.pending.pop_front();
+        ++sl.inflight;
+        lvbgs_ctx* c = sl.ctx;
+        View v = c->v;
+        v.frame = r.img;
+        v.out_mask = r.mask;
+        v.lr = r.lr;
+        v.seq_limit = (uint32_t)s->opt.seq_limit;
+        hv[m] = v;
+        ++c->apply_count;
+        uint32_t g[kNumGrids];
+        kernel_grids(c->v, s->opt, g);
+        for (int k = 0; k < kNumGrids; ++k)
+            hp[(size_t)k * (n + 1) + m + 1] = hp[(size_t)k * (n + 1) + m] + g[k];
+        q.frames.emplace_back(members[(size_t)m], r.tag);
+    }
+    if (hipMemcpyAsync(s->d_views, hv, sizeof(View) * (size_t)n, hipMemcpyHostToDevice, s->stream) != hipSuccess ||
+        hipMemcpyAsync(s->d_prefix, hp, sizeof(uint32_t) * kNumGrids * (size_t)(n + 1), hipMemcpyHostToDevice,
+                       s->stream) != hipSuccess ||
+        hipEventRecord(s->stage_ev[(size_t)si], s->stream) != hipSuccess)
+        return srv_err(s, LVBGS_ERR_HIP, "step upload");
+    s->stage_live[(size_t)si] = true;
+    const uint32_t* prefix[kNumGrids];
+    uint32_t total[kNumGrids];
+    for (int k = 0; k < kNumGrids; ++k) {
+        prefix[k] = s->d_prefix + (size_t)k * (n + 1);
+        total[k] = hp[(size_t)k * (n + 1) + n];
+    }
+    batch_step(s->d_views, n, prefix, total, s->opt, true, s->stream);
+    q.done = s->ev_free.back();
+    s->ev_free.pop_back();
+    if (hipEventRecord(q.done, s->stream) != hipSuccess || hipGetLastError() != hipSuccess) {
+        s->ev_free.push_back(q.done);
+        return srv_err(s, LVBGS_ERR_HIP, "step launch");
+    }
+    s->queued.push_back(std::move(q));
+    ++s->st.steps;
+    s->st.frames_launched += (uint64_t)n;
+    return LVBGS_OK;
+}
+
+extern "C" int lvbgs_server_pump(lvbgs_server* s) {
+    if (!s) return LVBGS_ERR_ARG;
+    hipSetDevice(s->device);
+    int rc = harvest(s, false);
+    if (rc) return rc;
+    int launched = 0;
+    while ((int)s->queued.size() < s->o.max_inflight) {
+        const uint64_t before = s->st.steps;
+        if ((rc = launch_step(s))) return rc;
+        if (s->st.steps == before) break;
+        ++launched;
+    }
+    return launched;
+}
+
+extern "C" int lvbgs_server_poll(lvbgs_server* s, lvbgs_done* out, int cap) {
+    if (!s || (!out && cap > 0) || cap < 0) return LVBGS_ERR_ARG;
+    const int n = std::min(cap, (int)s->done.size());
+    if (n > 0) {
+        std::memcpy(out, s->done.data(), sizeof(lvbgs_done) * (size_t)n);
+        s->done.erase(s->done.begin(), s->done.begin() + n);
+    }
+    return n;
+}
+
+extern "C" int lvbgs_server_drain(lvbgs_server* s) {
+    if (!s) return LVBGS_ERR_ARG;
+    hipSetDevice(s->device);
+    while (true) {
+        int rc = lvbgs_server_pump(s);
+        if (rc < 0) return rc;
+        if (s->queued.empty()) {
+            bool waiting = false;
+            for (auto& sl : s->slots) waiting = waiting || (sl.open && !sl.pending.empty());
+            if (!waiting) break;
+            continue;
+        }
+        if ((rc = harvest(s, true))) return rc;
+    }
+    return harvest(s, false);
+}
+
+extern "C" int lvbgs_server_open(lvbgs_server* s, const lvbgs_params* params, const void* d_img0, int width,
+                                 int height, int channels, int* stream_out) {
+    if (!s || !params || !d_img0 || !stream_out || width <= 0 || height <= 0 || (channels != 1 && channels != 3))
+        return s ? srv_err(s, LVBGS_ERR_ARG, "invalid stream parameters") : LVBGS_ERR_ARG;
+    hipSetDevice(s->device);
+    lvbgs_params p = *params;
+    p.device = s->device;
+    const size_t need = lvbgs_arena_bytes(LVBGS_SUBSENSE, p, width, height, channels);
+    // a pooled context of the same geometry and model parameters keeps its arena (no allocation)
+    lvbgs_ctx* c = nullptr;
+    for (size_t i = 0; i < s->pool.size(); ++i)
+        if (s->pool[i]->arena_size == need && s->pool[i]->W == width && s->pool[i]->H == height &&
+            s->pool[i]->C == channels && same_model_params(s->pool[i]->p, p)) {
+            c = s->pool[i];
+            s->pool.erase(s->pool.begin() + (long)i);
+            break;
+        }
+    if (!c) {
+        while (s->used + need > s->o.hbm_budget && !s->pool.empty()) {
+            s->used -= s->pool.back()->arena_size;
+            lvbgs_destroy(s->pool.back());
+            s->pool.pop_back();
+        }
+        if (s->used + need > s->o.hbm_budget) return srv_err(s, LVBGS_ERR_BUDGET, "model does not fit the budget");
+        const int rc = lvbgs_create(LVBGS_SUBSENSE, &p, &c);
+        if (rc) return srv_err(s, rc, "invalid stream parameters");
+    } else {
+        c->p.rng_seed = p.rng_seed;
+        c->p.stream_id = p.stream_id;
+    }
+    const size_t before = c->arena_size;
+    const int rc = lvbgs_initialize_device(c, d_img0, width, height, channels, nullptr, 0);
+    if (rc) {
+        s->err = lvbgs_last_error(c);
+        s->used -= before;
+        lvbgs_destroy(c);
+        return rc;
+    }
+    s->used += c->arena_size - before;
+    int id;
+    if (!s->free_ids.empty()) {
+        id = s->free_ids.back();
+        s->free_ids.pop_back();
+    } else {
+        id = (int)s->slots.size();
+        s->slots.emplace_back();
+    }
+    Slot& sl = s->slots[(size_t)id];
+    sl = Slot();
+    sl.ctx = c;
+    sl.open = true;
+    *stream_out = id;
+    return LVBGS_OK;
+}
+
+extern "C" int lvbgs_server_close(lvbgs_server* s, int stream) {
+    if (!s || stream < 0 || stream >= (int)s->slots.size() || !s->slots[(size_t)stream].open ||
+        s->slots[(size_t)stream].closing)
+        return s ? srv_err(s, LVBGS_ERR_ARG, "unknown stream") : LVBGS_ERR_ARG;
+    s->slots[(size_t)stream].closing = true;
+    return harvest(s, false);
+}
+
+extern "C" int lvbgs_server_submit(lvbgs_server* s, int stream, const void* d_img, void* d_mask, double lr,
+                                   uint64_t tag) {
+    if (!s || stream < 0 || stream >= (int)s->slots.size() || !s->slots[(size_t)stream].open ||
+        s->slots[(size_t)stream].closing || !d_img)
+        return s ? srv_err(s, LVBGS_ERR_ARG, "unknown or closing stream, or null frame") : LVBGS_ERR_ARG;
+    s->slots[(size_t)stream].pending.push_back(Request{(const uint8_t*)d_img, (uint8_t*)d_mask, lr, tag});
+    ++s->st.frames_submitted;
+    return LVBGS_OK;
+}
+
+extern "C" lvbgs_ctx* lvbgs_server_context(lvbgs_server* s, int stream) {
+    if (!s || stream < 0 || stream >= (int)s->slots.size() || !s->slots[(size_t)stream].open) return nullptr;
+    return s->slots[(size_t)stream].ctx;
+}
+
+extern "C" int lvbgs_server_get_stats(lvbgs_server* s, lvbgs_server_stats* out) {
+    if (!s || !out) return LVBGS_ERR_ARG;
+    lvbgs_server_stats st = s->st;
+    st.bytes_used = s->used;
+    st.open_streams = 0;
+    st.pending_frames = 0;
+    for (auto& sl : s->slots) {
+        st.open_streams += sl.open ? 1u : 0u;
+        st.pending_frames += (uint64_t)sl.pending.size();
+    }
+    st.pooled_contexts = (uint32_t)s->pool.size();
+    st.queued_steps = (uint32_t)s->queued.size();
+    *out = st;
+    return LVBGS_OK;
+}
+
+extern "C" void lvbgs_server_destroy(lvbgs_server* s) {
+    if (!s) return;
+    hipSetDevice(s->device);
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
+    server_release(s);
+    delete s;
+}

@@ -29,8 +29,8 @@ struct View {
     const uint8_t* roi;      // W*H, 0 = outside
     uint8_t* last_color;     // W*H*C
     uint16_t* last_desc;     // W*H*C
-    void* sA;                // N sample planes: C==3 -> u64 {d0,d1,d2,c0,c1}; C==1 -> u16 {d0}
-    uint8_t* sB;             // N sample planes: C==3 -> u8 {c2}; C==1 -> u8 {c0}
+    void* sA;                // N sample planes of one record per pixel: C==3 -> 3 x u32
+                             // {d0 | d1<<16, d2 | c0<<16 | c1<<24, c2}; C==1 -> u32 {d0 | c0<<16}
     // SuBSENSE feedback maps (f32 W*H)
     float *T, *R, *V, *MLD, *DminLT, *DminST, *RawLT, *RawST, *FinLT, *FinST;
     // u8 W*H maps
