@@ -145,7 +145,7 @@ static void carve(lvbgs_ctx* c, Arena& ar) {
     v.roi = ar.take<uint8_t>(npx);
     v.last_color = ar.take<uint8_t>(npx * C);
     v.last_desc = ar.take<uint16_t>(npx * C);
-    v.sA = ar.take<uint8_t>(N * npx * (C == 3 ? 12 : 4));
+    v.sA = ar.take<uint8_t>(N * ((npx + 63) / 64 * 64) * (C == 3 ? 12 : 4));  // tile-major, 64-pixel groups
     float** maps[] = {&v.T, &v.R, &v.V, &v.MLD, &v.DminLT, &v.DminST, &v.RawLT, &v.RawST, &v.FinLT, &v.FinST};
     for (auto m : maps) *m = sub ? ar.take<float>(npx) : nullptr;
     uint8_t** u8s[] = {&v.unstable, &v.blinks, &v.last_raw, &v.last_raw_blink, &v.last_fg, &v.last_dil_inv,
@@ -597,14 +597,16 @@ extern "C" long long lvbgs_get_state(lvbgs_ctx* c, const char* name, void* out, 
     if (n == "samples_color" || n == "samples_desc") {
         // unpack the sample planes to the oracle layout: N x (H x W x C)
         const size_t rec = C == 3 ? 12 : 4;
-        std::vector<uint8_t> a(N * npx * rec);
+        std::vector<uint8_t> a(N * ((npx + 63) / 64 * 64) * rec);
         if (hipMemcpy(a.data(), v.sA, a.size(), hipMemcpyDeviceToHost) != hipSuccess) return LVBGS_ERR_HIP;
         const bool col = n == "samples_color";
         bytes = N * npx * C * (col ? 1 : 2);
         if (bytes > cap) return LVBGS_ERR_ARG;
         for (size_t i = 0; i < N * npx; ++i) {
+            const size_t s = i / npx, p = i % npx;  // output order: sample-major, then pixel
+            const size_t src = ((p >> 6) * N + s) * 64 + (p & 63);
             uint32_t w[3] = {0, 0, 0};
-            std::memcpy(w, a.data() + i * rec, rec);
+            std::memcpy(w, a.data() + src * rec, rec);
             if (C == 3) {
                 if (col) {
                     uint8_t* o = (uint8_t*)out + i * 3;
@@ -845,7 +847,7 @@ extern "C" int lvbgs_batch_default_options(lvbgs_batch_options* o) {
     o->fixpoint_launches = 3;
     o->fixpoint_blocks = 256;
     o->fix_px_per_block = 8192;
-    o->fetch_depth = 4;
+    o->fetch_depth = 2;
     return LVBGS_OK;
 }
 

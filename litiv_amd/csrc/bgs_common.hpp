@@ -58,18 +58,19 @@ constexpr int kLbspDX[16] = {-2, 2, 0, 0, -2, 2, 2, -2, 0, -1, 0, 1, -1, 1, 1, -
 constexpr int kLbspDY[16] = {0, 0, -2, 2, 2, -2, 2, -2, 1, 0, -1, 0, -1, 1, -1, 1};
 
 // bit n = (|v[n]-ref| > t), LBSP.hpp:193-224. Packed taps: word w holds taps 4w..4w+3 (byte k = tap 4w+k).
+// v_sad_u8(v, ref, ~t) = |v-ref| + 2^32-1-t wraps past 2^32 (bit 31 clears) exactly when |v-ref| > t
+// (both at most 255); v_alignbit shifts that bit 31 into the result, tap 15 first, so bit n ends
+// up holding tap n's "|v-ref| <= t"; the result is its complement. Two VALU per tap.
 LV_D uint32_t lbsp_threshold_packed(const uint32_t taps[4], uint32_t ref, uint32_t t) {
+    const uint32_t acc = ~t;
     uint32_t d = 0;
 #pragma unroll
-    for (int w = 0; w < 4; ++w) {
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const uint32_t v = (taps[w] >> (8 * k)) & 0xFFu;
-            const uint32_t ad = __builtin_amdgcn_sad_u8(v, ref, 0u);  // |v-ref|
-            d |= (ad > t ? 1u : 0u) << (4 * w + k);
-        }
+    for (int n = 15; n >= 0; --n) {
+        const uint32_t v = (taps[n >> 2] >> (8 * (n & 3))) & 0xFFu;
+        const uint32_t x = __builtin_amdgcn_sad_u8(v, ref, acc);
+        d = __builtin_amdgcn_alignbit(d, x, 31u);  // (d << 1) | (x >> 31)
     }
-    return d;
+    return ~d & 0xFFFFu;
 }
 
 // ------------------------------------------------------------------------------------------

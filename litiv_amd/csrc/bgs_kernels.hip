@@ -78,10 +78,17 @@ struct alignas(4) Rec12 {
     uint32_t w0, w1, w2;
 };
 
+// Tile-major sample layout: the N samples of each group of 64 consecutive pixels are contiguous
+// (group g, sample s, lane p % 64), so a wavefront's ordered scan walks one compact region instead
+// of N planes a frame apart.
+LV_D size_t sample_index(const View& v, uint32_t s, uint32_t p) {
+    return ((size_t)(p >> 6) * (uint32_t)v.N + s) * 64u + (p & 63u);
+}
+
 // raw sample in registers: ra = record words 0-1 (RGB) or word 0 (grey), rb = word 2 (RGB)
 template <int C>
 LV_D void fetch_raw(const View& v, uint32_t s, uint32_t p, uint64_t& ra, uint32_t& rb) {
-    const size_t idx = (size_t)s * v.npx + p;
+    const size_t idx = sample_index(v, s, p);
     if constexpr (C == 3) {
         const Rec12 r = ((const Rec12*)v.sA)[idx];
         ra = (uint64_t)r.w0 | ((uint64_t)r.w1 << 32);
@@ -129,7 +136,7 @@ LV_D void load_sample(const View& v, uint32_t s, uint32_t p, uint32_t col[C], ui
 
 template <int C>
 LV_D void store_sample(const View& v, uint32_t s, uint32_t p, const uint32_t col[C], const uint32_t desc[C]) {
-    const size_t idx = (size_t)s * v.npx + p;
+    const size_t idx = sample_index(v, s, p);
     uint64_t ra;
     uint32_t rb;
     encode_raw<C>(col, desc, ra, rb);
@@ -965,14 +972,17 @@ LV_D void fixpoint_tail_body(const View& v, uint32_t bid, uint32_t nblk, uint32_
 // SuBSENSE feedback maps (SUB:396-420, :451-480; 1ch :254-329), raw mask and blink bookkeeping
 // (SUB:536-539). Whole image: pixels outside the ROI get raw = 0.
 // ----------------------------------------------------------------------------------------------
+// One pixel (p < npx). Returns p's samples examined (k_p) and its RawST / MLD after the update
+// (the values the model update's ghost test reads).
 template <int C>
-LV_D void feedback_body(const View& v, uint32_t bid, uint32_t nblk) {
+LV_D uint32_t feedback_pixel(const View& v, uint32_t p, float& rst_out, float& mld_out) {
     const uint8_t* frame = v.frame;
-    const uint32_t p = bid * TPB + threadIdx.x;
-    const bool valid = p < v.npx;
     uint32_t examined = 0;
     uint8_t raw = 0;
-    if (valid && v.roi[p]) {
+    if (!v.roi[p]) {
+        rst_out = v.RawST[p];
+        mld_out = v.MLD[p];
+    } else {
         const uint32_t frame_no = cur_frame(v);
         const uint32_t mvn = v.mv_avgs;
         const float aLT = 1.0f / (float)(frame_no < mvn ? frame_no : mvn);
@@ -1019,6 +1029,8 @@ LV_D void feedback_body(const View& v, uint32_t bid, uint32_t nblk) {
         v.DminST[p] = dst;
         v.RawLT[p] = rlt;
         v.RawST[p] = rst;
+        rst_out = rst;
+        mld_out = mld;
         const float lcap = v.sc->lower_cap, ucap = v.sc->upper_cap;
         const float mn = fmin_ref(dlt, dst), mx = fmax_ref(dlt, dst);
         const bool last_fg = v.last_fg[p] != 0;
@@ -1053,14 +1065,12 @@ LV_D void feedback_body(const View& v, uint32_t bid, uint32_t nblk) {
             v.last_desc[(size_t)p * C + c] = (uint16_t)intra[c];
         }
     }
-    if (valid) {
-        const uint8_t cb = raw ^ v.last_raw[p];
-        v.blinks[p] = cb | v.last_raw_blink[p];
-        v.last_raw_blink[p] = cb;
-        v.last_raw[p] = raw;
-        v.raw[p] = raw;
-    }
-    block_add<TPB>(examined, v.sc->ex_part);  // workload counter (sum of k_p)
+    const uint8_t cb = raw ^ v.last_raw[p];
+    v.blinks[p] = cb | v.last_raw_blink[p];
+    v.last_raw_blink[p] = cb;
+    v.last_raw[p] = raw;
+    v.raw[p] = raw;
+    return examined;
 }
 
 // LOBSTER: raw mask (LOB:496-497, :554-555) and previous-raw bookkeeping for the next speculation
@@ -1083,7 +1093,9 @@ LV_D void lobster_raw_body(const View& v, uint32_t bid, uint32_t nblk) {
 // model update: every pixel of the inner rectangle gathers the writes aimed at its samples, in
 // the raster order of the writers; the last write of each slot wins (SUB:406-449, LOB:498-512).
 // ----------------------------------------------------------------------------------------------
-template <int C, int ALGO>
+// FUSED (SuBSENSE): the feedback maps of the thread's pixel are updated first, in the same pass
+// (they depend on the pixel alone), and the ghost test uses the new values from registers.
+template <int C, int ALGO, bool FUSED>
 LV_D void update_body(const View& v, uint32_t bid, uint32_t nblk) {
     const uint8_t* frame = v.frame;
     constexpr int LW = TW + 4, LH = TH + 4;
@@ -1103,12 +1115,16 @@ LV_D void update_body(const View& v, uint32_t bid, uint32_t nblk) {
     }
     __syncthreads();
     const int lx = threadIdx.x % TW, ly = threadIdx.x / TW, x = x0 + lx, y = y0 + ly;
-    uint32_t nw = 0;
+    uint32_t nw = 0, examined = 0;
+    float rst = 0.f, mld = 1.f;
+    if constexpr (FUSED)
+        if (x < v.W && y < v.H) examined = feedback_pixel<C>(v, (uint32_t)(y * v.W + x), rst, mld);
     if (x >= 2 && y >= 2 && x < v.W - 2 && y < v.H - 2) {
         const uint32_t p = (uint32_t)(y * v.W + x);
         // ghost test of writers at or after p reads p's updated maps (SUB:438-441)
         bool ghost_new = false;
-        if constexpr (ALGO == kSuBSENSE) ghost_new = v.RawST[p] > 0.995f && v.MLD[p] < 0.010f;
+        if constexpr (FUSED) ghost_new = rst > 0.995f && mld < 0.010f;
+        else if constexpr (ALGO == kSuBSENSE) ghost_new = v.RawST[p] > 0.995f && v.MLD[p] < 0.010f;
         // Writes landing on p, as bits in sequential order: writer j of the 5x5 window (raster
         // order) owns bit 2j (its self update, j == 12 only) and bit 2j+1 (its diffusion into p).
         uint64_t wm = 0;
@@ -1148,6 +1164,7 @@ LV_D void update_body(const View& v, uint32_t bid, uint32_t nblk) {
         }
     }
     block_add<TPB>(nw, v.sc->wr_part);
+    if constexpr (FUSED) block_add<TPB>(examined, v.sc->ex_part);  // workload counter (sum of k_p)
 }
 
 // LOBSTER: last colour frame := input (LOB:580)
@@ -1745,8 +1762,10 @@ LV_D void for_batch_block(const BatchArgs& b, F&& f) {
 
 template <int C, int ALGO>
 __global__ void __launch_bounds__(TPB) k_describe(View v) { describe_body<C, ALGO>(v, LV_SINGLE_BID); }
+// occupancy 5 fits fetch depth 2 without spilling (at most 96 VGPRs); depth 4 stays at 4
+#define LV_MATCH_WAVES(KF) __attribute__((amdgpu_waves_per_eu((KF) == 2 ? 5 : 4)))
 template <int C, int ALGO, int KF>
-__global__ void __launch_bounds__(TPB) k_match(View v) { match_body<C, ALGO, KF>(v, LV_SINGLE_BID); }
+__global__ void __launch_bounds__(TPB) LV_MATCH_WAVES(KF) k_match(View v) { match_body<C, ALGO, KF>(v, LV_SINGLE_BID); }
 template <int C, int ALGO>
 __global__ void __launch_bounds__(TPB) k_match_cont(View v) { match_cont_body<C, ALGO>(v, LV_SINGLE_BID); }
 template <int C, int ALGO>
@@ -1755,11 +1774,10 @@ template <int C, int ALGO>
 __global__ void __launch_bounds__(256) k_fixpoint_tail(View v, uint32_t k0, uint32_t cap) {
     fixpoint_tail_body<C, ALGO>(v, 0u, 1u, k0, cap);
 }
-template <int C>
-__global__ void __launch_bounds__(TPB) k_feedback(View v) { feedback_body<C>(v, LV_SINGLE_BID); }
+
 __global__ void __launch_bounds__(TPB) k_lobster_raw(View v) { lobster_raw_body(v, LV_SINGLE_BID); }
-template <int C, int ALGO>
-__global__ void __launch_bounds__(TPB) k_update(View v) { update_body<C, ALGO>(v, LV_SINGLE_BID); }
+template <int C, int ALGO, bool FUSED>
+__global__ void __launch_bounds__(TPB) k_update(View v) { update_body<C, ALGO, FUSED>(v, LV_SINGLE_BID); }
 template <int C>
 __global__ void __launch_bounds__(TPB) k_copy_last_color(View v) { copy_last_color_body<C>(v, LV_SINGLE_BID); }
 __global__ void __launch_bounds__(TPB) k_close(View v) { close_body(v, LV_SINGLE_BID); }
@@ -1788,7 +1806,7 @@ __global__ void __launch_bounds__(TPB) k_refresh(View v, int use_pending, float 
     })
 __global__ void __launch_bounds__(TPB) kb_describe(BatchArgs b) { LV_BATCH_C(describe_body, kSuBSENSE); }
 template <int KF>
-__global__ void __launch_bounds__(TPB) kb_match(BatchArgs b) { LV_BATCH_C(match_body, kSuBSENSE, KF); }
+__global__ void __launch_bounds__(TPB) LV_MATCH_WAVES(KF) kb_match(BatchArgs b) { LV_BATCH_C(match_body, kSuBSENSE, KF); }
 __global__ void __launch_bounds__(TPB) kb_match_cont(BatchArgs b) { LV_BATCH_C(match_cont_body, kSuBSENSE); }
 __global__ void __launch_bounds__(TPB) kb_fixpoint(BatchArgs b, uint32_t k) {
     for_batch_block(b, [&](const View& v, uint32_t bid, uint32_t nblk) {
@@ -1802,8 +1820,7 @@ __global__ void __launch_bounds__(256) kb_fixpoint_tail(BatchArgs b, uint32_t k0
         else fixpoint_tail_body<1, kSuBSENSE>(v, bid, nblk, k0, cap);
     });
 }
-__global__ void __launch_bounds__(TPB) kb_feedback(BatchArgs b) { LV_BATCH_C(feedback_body); }
-__global__ void __launch_bounds__(TPB) kb_update(BatchArgs b) { LV_BATCH_C(update_body, kSuBSENSE); }
+__global__ void __launch_bounds__(TPB) kb_update(BatchArgs b) { LV_BATCH_C(update_body, kSuBSENSE, true); }
 __global__ void __launch_bounds__(TPB) kb_close(BatchArgs b) { for_batch_block(b, close_body); }
 __global__ void __launch_bounds__(TPB) kb_ccl_local(BatchArgs b) { for_batch_block(b, ccl_local_body); }
 __global__ void __launch_bounds__(TPB) kb_ccl_border(BatchArgs b) { for_batch_block(b, ccl_border_body); }
@@ -1858,8 +1875,7 @@ static void frame_step_t(const View& v, const StepOptions& opt, hipStream_t st) 
     hipLaunchKernelGGL((k_fixpoint_tail<C, ALGO>), dim3(1), dim3(256), 0, st, v, (uint32_t)opt.fixpoint_launches + 1u,
                        (uint32_t)opt.tail_cap);
     if constexpr (ALGO == kSuBSENSE) {
-        hipLaunchKernelGGL((k_feedback<C>), dim3(g[kGridPix]), dim3(TPB), 0, st, v);
-        hipLaunchKernelGGL((k_update<C, ALGO>), dim3(g[kGridTile]), dim3(TPB), 0, st, v);
+        hipLaunchKernelGGL((k_update<C, ALGO, true>), dim3(g[kGridTile]), dim3(TPB), 0, st, v);
         hipLaunchKernelGGL(k_close, dim3(g[kGridPost]), dim3(TPB), 0, st, v);
         hipLaunchKernelGGL(k_ccl_local, dim3(g[kGridCcl]), dim3(TPB), 0, st, v);
         hipLaunchKernelGGL(k_ccl_border, dim3(g[kGridCclBorder]), dim3(TPB), 0, st, v);
@@ -1871,7 +1887,7 @@ static void frame_step_t(const View& v, const StepOptions& opt, hipStream_t st) 
         hipLaunchKernelGGL((k_refresh<C, ALGO>), dim3(g[kGridSweep]), dim3(TPB), 0, st, v, 1, 0.f, 0, 0u, 0u);
     } else {
         hipLaunchKernelGGL(k_lobster_raw, dim3(g[kGridPix]), dim3(TPB), 0, st, v);
-        hipLaunchKernelGGL((k_update<C, ALGO>), dim3(g[kGridTile]), dim3(TPB), 0, st, v);
+        hipLaunchKernelGGL((k_update<C, ALGO, false>), dim3(g[kGridTile]), dim3(TPB), 0, st, v);
         hipLaunchKernelGGL((k_median<false>), dim3(g[kGridPost]), dim3(TPB), 0, st, v);
         if (v.out_mask) hipLaunchKernelGGL(k_copy_mask, dim3(g[kGridPix]), dim3(TPB), 0, st, v);
         hipLaunchKernelGGL((k_copy_last_color<C>), dim3(cdiv((size_t)v.npx * C, TPB)), dim3(TPB), 0, st, v);
@@ -1906,7 +1922,6 @@ void batch_step(const View* d_views, int ns, const uint32_t* const prefix[kNumGr
         hipLaunchKernelGGL(kb_fixpoint, dim3(total[kGridFix]), dim3(TPB), 0, st, args(kGridFix), k);
     hipLaunchKernelGGL(kb_fixpoint_tail, dim3(total[kGridOne]), dim3(256), 0, st, args(kGridOne),
                        (uint32_t)opt.fixpoint_launches + 1u, (uint32_t)opt.tail_cap);
-    hipLaunchKernelGGL(kb_feedback, dim3(total[kGridPix]), dim3(TPB), 0, st, args(kGridPix));
     hipLaunchKernelGGL(kb_update, dim3(total[kGridTile]), dim3(TPB), 0, st, args(kGridTile));
     hipLaunchKernelGGL(kb_close, dim3(total[kGridPost]), dim3(TPB), 0, st, args(kGridPost));
     hipLaunchKernelGGL(kb_ccl_local, dim3(total[kGridCcl]), dim3(TPB), 0, st, args(kGridCcl));
