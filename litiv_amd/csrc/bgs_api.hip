@@ -145,7 +145,8 @@ static void carve(lvbgs_ctx* c, Arena& ar) {
     v.roi = ar.take<uint8_t>(npx);
     v.last_color = ar.take<uint8_t>(npx * C);
     v.last_desc = ar.take<uint16_t>(npx * C);
-    v.sA = ar.take<uint8_t>(N * ((npx + 63) / 64 * 64) * (C == 3 ? 12 : 4));  // tile-major, 64-pixel groups
+    // tile-major, 64-pixel groups, padded for loads past the last sample of the last group
+    v.sA = ar.take<uint8_t>((N * ((npx + 63) / 64) + kMaxFetch) * 64 * (C == 3 ? 12 : 4));
     float** maps[] = {&v.T, &v.R, &v.V, &v.MLD, &v.DminLT, &v.DminST, &v.RawLT, &v.RawST, &v.FinLT, &v.FinST};
     for (auto m : maps) *m = sub ? ar.take<float>(npx) : nullptr;
     uint8_t** u8s[] = {&v.unstable, &v.blinks, &v.last_raw, &v.last_raw_blink, &v.last_fg, &v.last_dil_inv,

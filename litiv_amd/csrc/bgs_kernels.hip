@@ -99,6 +99,23 @@ LV_D void fetch_raw(const View& v, uint32_t s, uint32_t p, uint64_t& ra, uint32_
     }
 }
 
+// records of one pixel: sample s at rec_base(v, p) + s * 64 (tile-major layout)
+LV_D const uint32_t* rec_base(const View& v, uint32_t p) {
+    return (const uint32_t*)v.sA + sample_index(v, 0u, p) * (v.C == 3 ? 3u : 1u);
+}
+
+template <int C>
+LV_D void fetch_at(const uint32_t* rec, uint64_t& ra, uint32_t& rb) {
+    if constexpr (C == 3) {
+        const Rec12 r = *(const Rec12*)rec;
+        ra = (uint64_t)r.w0 | ((uint64_t)r.w1 << 32);
+        rb = r.w2;
+    } else {
+        ra = *rec;
+        rb = 0;
+    }
+}
+
 template <int C>
 LV_D void decode_raw(uint64_t ra, uint32_t rb, uint32_t col[C], uint32_t desc[C]) {
     if constexpr (C == 3) {
@@ -450,13 +467,16 @@ LV_D void scan_seq(const View& v, const uint8_t* lut, const PxCtx<C>& px, int x,
     const uint32_t REQ = (uint32_t)v.REQ;
     const ReplLds& R = repl_lds();
     const int t = (int)threadIdx.x;
+    constexpr uint32_t kStride = 64u * (C == 3 ? 3u : 1u);  // words between samples of a pixel
+    // Unconditional loads (the sample planes are padded by kMaxFetch samples) at constant offsets
+    // from a running pointer keep all KF loads in flight together without address arithmetic.
+    const uint32_t* cur = rec_base(v, p) + st.s * kStride;
     while (st.s < limit && st.good < REQ) {
         uint64_t ra[KF];
         uint32_t rb[KF];
-        const uint32_t s0 = st.s;
 #pragma unroll
-        for (int j = 0; j < KF; ++j)
-            if (s0 + j < limit) fetch_raw<C>(v, s0 + j, p, ra[j], rb[j]);
+        for (int j = 0; j < KF; ++j) fetch_at<C>(cur + j * kStride, ra[j], rb[j]);
+        cur += KF * kStride;
 #pragma unroll
         for (int j = 0; j < KF; ++j) {
             if (st.s >= limit || st.good >= REQ) break;
@@ -1098,27 +1118,52 @@ LV_D void lobster_raw_body(const View& v, uint32_t bid, uint32_t nblk) {
 template <int C, int ALGO, bool FUSED>
 LV_D void update_body(const View& v, uint32_t bid, uint32_t nblk) {
     const uint8_t* frame = v.frame;
-    constexpr int LW = TW + 4, LH = TH + 4;
-    __shared__ uint32_t s_rec[LH][LW];  // records of the tile's 5x5 neighbourhood, tgt = kRecNone when
-    __shared__ uint8_t s_bg[LH][LW];    // the writer is outside the image; s_bg: writer decided background
+    constexpr int LW = TW + 4, LH = TH + 4, NT = (LW * LH + TPB - 1) / TPB;
+    // the tile's 5x5 neighbourhood: records (tgt = kRecNone outside the image), "writer decided
+    // background", and the writers' current colour and intra descriptor (the values they write)
+    __shared__ uint32_t s_rec[LH][LW];
+    __shared__ uint8_t s_bg[LH][LW];
+    __shared__ uint8_t s_col[LH][LW * C];
+    __shared__ uint16_t s_desc[LH][LW * C];
     const int ntx = (v.W + TW - 1) / TW, nty = (v.H + TH - 1) / TH;
     const int tid = (int)bid;
     const int x0 = (tid % ntx) * TW, y0 = (tid / ntx) * TH;
-    for (int i = threadIdx.x; i < LW * LH; i += TPB) {
+    // neighbourhood loads first, into registers: their latency overlaps the feedback pass below
+    uint32_t t_rec[NT], t_col[NT][C], t_desc[NT][C];
+    uint8_t t_dec[NT];
+#pragma unroll
+    for (int k = 0; k < NT; ++k) {
+        const int i = (int)threadIdx.x + k * TPB;
         const int ly = i / LW, lx = i % LW, gx = x0 + lx - 2, gy = y0 + ly - 2;
-        const bool in = gx >= 0 && gx < v.W && gy >= 0 && gy < v.H;
+        const bool in = i < LW * LH && gx >= 0 && gx < v.W && gy >= 0 && gy < v.H;
         const size_t q = in ? (size_t)gy * v.W + gx : 0;
-        const uint32_t r = in ? v.rec[q] : kRecNone;  // both loads issued together
-        const uint8_t d = v.dec[q];
-        s_rec[ly][lx] = r;
-        s_bg[ly][lx] = (rec_tgt(r) != kRecNone && d == 0) ? 1 : 0;
+        t_rec[k] = in ? v.rec[q] : kRecNone;
+        t_dec[k] = v.dec[q];
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+            t_col[k][c] = frame[q * C + c];
+            t_desc[k][c] = v.desc_cur[q * C + c];
+        }
     }
-    __syncthreads();
     const int lx = threadIdx.x % TW, ly = threadIdx.x / TW, x = x0 + lx, y = y0 + ly;
     uint32_t nw = 0, examined = 0;
     float rst = 0.f, mld = 1.f;
     if constexpr (FUSED)
         if (x < v.W && y < v.H) examined = feedback_pixel<C>(v, (uint32_t)(y * v.W + x), rst, mld);
+#pragma unroll
+    for (int k = 0; k < NT; ++k) {
+        const int i = (int)threadIdx.x + k * TPB;
+        if (i >= LW * LH) break;
+        const int ly2 = i / LW, lx2 = i % LW;
+        s_rec[ly2][lx2] = t_rec[k];
+        s_bg[ly2][lx2] = (rec_tgt(t_rec[k]) != kRecNone && t_dec[k] == 0) ? 1 : 0;
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+            s_col[ly2][lx2 * C + c] = (uint8_t)t_col[k][c];
+            s_desc[ly2][lx2 * C + c] = (uint16_t)t_desc[k][c];
+        }
+    }
+    __syncthreads();
     if (x >= 2 && y >= 2 && x < v.W - 2 && y < v.H - 2) {
         const uint32_t p = (uint32_t)(y * v.W + x);
         // ghost test of writers at or after p reads p's updated maps (SUB:438-441)
@@ -1157,9 +1202,12 @@ LV_D void update_body(const View& v, uint32_t bid, uint32_t nblk) {
             else if (word == 1) done1 |= bit;
             else if (word == 2) done2 |= bit;
             else done3 |= bit;
-            const uint32_t q = (uint32_t)((y + dy) * v.W + (x + dx));
             uint32_t col[C], desc[C];
-            current_value<C>(v, frame, q, col, desc);
+#pragma unroll
+            for (int c = 0; c < C; ++c) {
+                col[c] = s_col[ly + 2 + dy][(lx + 2 + dx) * C + c];
+                desc[c] = s_desc[ly + 2 + dy][(lx + 2 + dx) * C + c];
+            }
             store_sample<C>(v, sl, p, col, desc);
         }
     }
@@ -1762,8 +1810,8 @@ LV_D void for_batch_block(const BatchArgs& b, F&& f) {
 
 template <int C, int ALGO>
 __global__ void __launch_bounds__(TPB) k_describe(View v) { describe_body<C, ALGO>(v, LV_SINGLE_BID); }
-// occupancy 5 fits fetch depth 2 without spilling (at most 96 VGPRs); depth 4 stays at 4
-#define LV_MATCH_WAVES(KF) __attribute__((amdgpu_waves_per_eu((KF) == 2 ? 5 : 4)))
+// occupancy 5 (at most 96 VGPRs) fits both fetch depths without spilling
+#define LV_MATCH_WAVES(KF) __attribute__((amdgpu_waves_per_eu(5)))
 template <int C, int ALGO, int KF>
 __global__ void __launch_bounds__(TPB) LV_MATCH_WAVES(KF) k_match(View v) { match_body<C, ALGO, KF>(v, LV_SINGLE_BID); }
 template <int C, int ALGO>

@@ -6,6 +6,10 @@
 
 namespace lvbgs {
 
+// largest sequential-match fetch depth; the sample planes are padded by this many samples so a
+// scan may load past a pixel's last sample
+constexpr int kMaxFetch = 4;
+
 struct StepOptions {
     int fixpoint_launches = 3;   // fix-point iterations launched as full-grid kernels
     int fixpoint_blocks = 256;   // grid of each fix-point kernel (grid-stride over the worklist)
