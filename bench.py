@@ -46,6 +46,37 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def measured_traffic(workload):
+    """HBM bytes per launch of the roofline kernel from the committed hardware-counter summary of
+    this workload (profiles/<workload>_traffic.json: FETCH_SIZE and WRITE_SIZE per dispatch, each
+    scaled by what the counter reads for a device copy of known size), or None."""
+    path = os.path.join(ROOT, "profiles", workload + "_traffic.json")
+    if not os.path.exists(path):
+        return None, None
+    with open(path) as f:
+        t = json.load(f)
+    return int(t["traffic_bytes_per_launch"]), "profiles/%s_traffic.json" % workload
+
+
+def traffic_fields(workload, algorithmic_bytes):
+    traffic, source = measured_traffic(workload)
+    out = {"traffic": traffic}
+    if traffic is not None:
+        out["traffic_source"] = source
+        out["traffic_over_algorithmic"] = round(traffic / algorithmic_bytes, 3)
+    return out
+
+
+def calibration_copy(torch, dev, mb):
+    """One device copy of known size (profiling runs scale hardware byte counters by it)."""
+    if mb:
+        a = torch.empty(mb << 20, dtype=torch.uint8, device=dev)
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
+
+
 def match_bytes(C, n_roi, examined):
     """Algorithmic bytes of one speculative-match pass over a stream: per ROI pixel the input frame (C),
     intra descriptors (2C), R (4), previous unstable flag (1), update record (4), previous raw decision
@@ -151,13 +182,7 @@ def run_batch(args, preset, n_streams, ref_frames):
     torch.cuda.synchronize()
     log("workload subsense_batch64: %d streams, %.2f Mpix per step, %d warm-up + %d timed steps, %.1f GB of frames"
         % (n_streams, px_step / 1e6, args.warmup, steps, sum(f.numel() for f in frames) / 1e9))
-    if args.calib_copy_mb:
-        # a device copy of known size, for scaling hardware byte counters in profiling runs
-        a = torch.empty(args.calib_copy_mb << 20, dtype=torch.uint8, device=dev)
-        b = torch.empty_like(a)
-        b.copy_(a)
-        torch.cuda.synchronize()
-        del a, b
+    calibration_copy(torch, dev, args.calib_copy_mb)
     f0 = [s.frame(0) for s in seqs]
     subs = [BackgroundSubtractorSuBSENSE(stream_id=i, device=0) for i in range(n_streams)]
 
@@ -269,10 +294,11 @@ def run_batch(args, preset, n_streams, ref_frames):
             "value": round(plain, 2), "unit": "Mpix/s",
             "what": "same 64 streams and frames through the one-stream drop-in, apply + synchronize per stream"},
         "batched_over_plain": None if plain is None else round(mpix / plain, 3),
-        "roofline": {"bound": "hbm", "kernel": "kb_match (speculative ordered sample match, all streams)",
-                     "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "bytes_per_launch": int(bytes_launch), "avg_launch_us": round(avg_match_s * 1e6, 2)},
+        "roofline": dict({"bound": "hbm", "kernel": "kb_match (speculative ordered sample match, all streams)",
+                          "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                          "frac": round(achieved / HBM_PEAK_GBS, 4)},
+                         **traffic_fields("subsense_batch64", bytes_launch),
+                         bytes_per_launch=int(bytes_launch), avg_launch_us=round(avg_match_s * 1e6, 2)),
         "frame_latency_ms": {"p50": round(float(np.percentile(step_ms, 50)), 4),
                              "p99": round(float(np.percentile(step_ms, 99)), 4),
                              "what": "GPU time of one batched step (every stream's frame), HIP events, %d steps"
@@ -464,6 +490,7 @@ def run_plain(args, name, algo, preset, ref_frames):
     seq.frames_device(0, total, frames.data_ptr())
     torch.cuda.synchronize()
     base = frames.data_ptr()
+    calibration_copy(torch, dev, args.calib_copy_mb)
     bgs = (BackgroundSubtractorSuBSENSE if algo == "subsense" else BackgroundSubtractorLOBSTER)(device=0)
     f0 = seq.frame(0)
 
@@ -531,11 +558,12 @@ def run_plain(args, name, algo, preset, ref_frames):
         "data": "synthetic (deterministic generator, frames resident in HBM)",
         "config": {"workload": name, "width": W, "height": H, "channels": C, "frames": total,
                    "algo": algo, "learning_rate": lr, "streams": 1, "parallelism": "single stream, one GPU"},
-        "roofline": {"bound": "hbm", "kernel": "k_match (speculative ordered sample match)",
-                     "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "bytes_per_launch": int(bytes_match), "avg_launch_us": round(avg_match_s * 1e6, 2),
-                     "examined_per_px": round(ex_spec / n_meas / n_roi, 3)},
+        "roofline": dict({"bound": "hbm", "kernel": "k_match (speculative ordered sample match)",
+                          "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                          "frac": round(achieved / HBM_PEAK_GBS, 4)},
+                         **traffic_fields(name, bytes_match),
+                         bytes_per_launch=int(bytes_match), avg_launch_us=round(avg_match_s * 1e6, 2),
+                         examined_per_px=round(ex_spec / n_meas / n_roi, 3)),
         "frame_latency_ms": {"p50": round(float(np.percentile(step_ms, 50)), 4),
                              "p99": round(float(np.percentile(step_ms, 99)), 4),
                              "what": "GPU time of one frame step (HIP events), %d frames" % n_meas},

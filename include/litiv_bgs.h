@@ -122,9 +122,10 @@ typedef struct lvbgs_stats {
     uint32_t fixpoint_iters;  /* fix-point iterations that had work in the last step */
     uint32_t max_fixpoint_iter;
     uint32_t width, height, channels, n_roi;
-    uint64_t examined_speculative; /* samples examined by the speculative match pass alone */
+    uint64_t examined_speculative; /* samples examined by the speculative match kernel alone */
     uint32_t fixpoint_work[8];     /* [0]: pixels finished by the cooperative match phase;
                                       [k]: worklist size of fix-point iteration k (1..7) */
+    uint64_t examined_cooperative; /* samples examined by the cooperative match phase */
 } lvbgs_stats;
 int lvbgs_get_stats(lvbgs_ctx* ctx, lvbgs_stats* out);
 

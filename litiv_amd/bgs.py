@@ -53,7 +53,7 @@ class Stats(C.Structure):
     _fields_ = [("examined", C.c_uint64), ("writes", C.c_uint64), ("fixpoint_iters", C.c_uint32),
                 ("max_fixpoint_iter", C.c_uint32), ("width", C.c_uint32), ("height", C.c_uint32),
                 ("channels", C.c_uint32), ("n_roi", C.c_uint32), ("examined_speculative", C.c_uint64),
-                ("fixpoint_work", C.c_uint32 * 8)]
+                ("fixpoint_work", C.c_uint32 * 8), ("examined_cooperative", C.c_uint64)]
 
 
 class BatchOptions(C.Structure):

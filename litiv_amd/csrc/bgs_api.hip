@@ -559,6 +559,7 @@ extern "C" int lvbgs_get_stats(lvbgs_ctx* c, lvbgs_stats* out) {
     out->examined = sum(s.ex_part);
     out->writes = sum(s.wr_part);
     out->examined_speculative = sum(s.spec_part);
+    out->examined_cooperative = sum(s.coop_part);
     for (int i = 0; i < 8; ++i) out->fixpoint_work[i] = s.wl_work[i];
     out->fixpoint_work[0] = s.cont_count;
     out->fixpoint_iters = s.iters;

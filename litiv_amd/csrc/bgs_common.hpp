@@ -149,7 +149,8 @@ struct alignas(16) Scalars {
     unsigned long long cd_part[64];   // frame-level colour difference sum (model reset, caps)
     unsigned long long ex_part[64];   // samples examined by the final match (k_p)
     unsigned long long wr_part[64];   // sample writes (w_p)
-    unsigned long long spec_part[64]; // samples examined by the speculative match pass
+    unsigned long long spec_part[64]; // samples examined by the speculative match kernel
+    unsigned long long coop_part[64]; // samples examined by the cooperative match phase
     uint32_t iters;              // per-frame: fix-point iterations that had work
     uint32_t max_iter_seen;      // largest fix-point iteration index that had work (sticky)
     uint32_t epoch;              // steps since allocation (worklist de-duplication tags)

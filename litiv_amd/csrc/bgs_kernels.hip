@@ -614,6 +614,7 @@ LV_D void describe_body(const View& v, uint32_t bid, uint32_t nblk) {
         v.sc->ex_part[threadIdx.x] = 0;
         v.sc->wr_part[threadIdx.x] = 0;
         v.sc->spec_part[threadIdx.x] = 0;
+        v.sc->coop_part[threadIdx.x] = 0;
     }
     load_frame_tile<C>(tile, frame, v.W, v.H, x0, y0);
     load_lut(lut, v.sc);
@@ -885,7 +886,7 @@ LV_D void match_cont_body(const View& v, uint32_t bid, uint32_t nblk) {
             if (fg != guess && forward_target(v, v.rec[p], x, y, t)) wl_push(v, t, 1u, epoch, v.wl1);
         }
     }
-    block_add<TPB>(examined, v.sc->spec_part);
+    block_add<TPB>(examined, v.sc->coop_part);
 }
 
 // fix-point re-match of worklisted pixels (all lanes call; kGroup lanes per item)
@@ -1914,10 +1915,10 @@ static void frame_step_t(const View& v, const StepOptions& opt, hipStream_t st) 
     if (opt.timed_stage == 1) (void)hipEventRecord(opt.ev_begin, st);
     if (opt.fetch_depth == 4) hipLaunchKernelGGL((k_match<C, ALGO, 4>), dim3(g[kGridTile]), dim3(TPB), 0, st, v);
     else hipLaunchKernelGGL((k_match<C, ALGO, 2>), dim3(g[kGridTile]), dim3(TPB), 0, st, v);
-    // the cooperative phase finishes what the sequential phase left (scans cut at seq_limit and
-    // pixels with more than two overwritten slots); it exits at once when nothing was left
-    hipLaunchKernelGGL((k_match_cont<C, ALGO>), dim3(g[kGridCont]), dim3(TPB), 0, st, v);
     if (opt.timed_stage == 1) (void)hipEventRecord(opt.ev_end, st);
+    // the cooperative phase finishes what the sequential phase left (scans cut at seq_limit and
+    // pixels with more than kRepl overwritten slots); it exits at once when nothing was left
+    hipLaunchKernelGGL((k_match_cont<C, ALGO>), dim3(g[kGridCont]), dim3(TPB), 0, st, v);
     for (uint32_t k = 1; k <= (uint32_t)opt.fixpoint_launches; ++k)
         hipLaunchKernelGGL((k_fixpoint<C, ALGO>), dim3(g[kGridFix]), dim3(TPB), 0, st, v, k);
     hipLaunchKernelGGL((k_fixpoint_tail<C, ALGO>), dim3(1), dim3(256), 0, st, v, (uint32_t)opt.fixpoint_launches + 1u,
@@ -1964,8 +1965,8 @@ void batch_step(const View* d_views, int ns, const uint32_t* const prefix[kNumGr
     if (opt.timed_stage == 1) (void)hipEventRecord(opt.ev_begin, st);
     if (opt.fetch_depth == 4) hipLaunchKernelGGL(kb_match<4>, dim3(total[kGridTile]), dim3(TPB), 0, st, args(kGridTile));
     else hipLaunchKernelGGL(kb_match<2>, dim3(total[kGridTile]), dim3(TPB), 0, st, args(kGridTile));
-    hipLaunchKernelGGL(kb_match_cont, dim3(total[kGridCont]), dim3(TPB), 0, st, args(kGridCont));
     if (opt.timed_stage == 1) (void)hipEventRecord(opt.ev_end, st);
+    hipLaunchKernelGGL(kb_match_cont, dim3(total[kGridCont]), dim3(TPB), 0, st, args(kGridCont));
     for (uint32_t k = 1; k <= (uint32_t)opt.fixpoint_launches; ++k)
         hipLaunchKernelGGL(kb_fixpoint, dim3(total[kGridFix]), dim3(TPB), 0, st, args(kGridFix), k);
     hipLaunchKernelGGL(kb_fixpoint_tail, dim3(total[kGridOne]), dim3(256), 0, st, args(kGridOne),
