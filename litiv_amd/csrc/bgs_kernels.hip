@@ -1229,52 +1229,45 @@ LV_D void copy_last_color_body(const View& v, uint32_t bid, uint32_t nblk) {
 // ----------------------------------------------------------------------------------------------
 constexpr int PW = 64, PH = 16;  // post-processing tile; 256 threads x 4 pixels
 
-// morphologyEx CLOSE with a 3x3 rectangle, neutral border: erode3(dilate3(raw))
-LV_D void close_body(const View& v, uint32_t bid, uint32_t nblk) {
-    constexpr int LW = PW + 4, LH = PH + 4;
-    __shared__ uint8_t s_raw[LH][LW];
-    __shared__ uint8_t s_dil[LH - 2][LW - 2];
-    const int ntx = (v.W + PW - 1) / PW, nty = (v.H + PH - 1) / PH;
-    const int tid = (int)bid;
-    const int x0 = (tid % ntx) * PW, y0 = (tid / ntx) * PH;
-    for (int i = threadIdx.x; i < LW * LH; i += TPB) {
-        const int ly = i / LW, lx = i % LW, gx = x0 + lx - 2, gy = y0 + ly - 2;
+// --- hole fill: union-find labelling of the 0-pixels of PreFlood (4-connectivity) ---
+constexpr int CT = 32;  // CCL tile (32x32)
+
+// One 32x32 tile: PreFlood = morphologyEx(raw, CLOSE, 3x3) = erode3(dilate3(raw)) with neutral
+// borders (SUB:540), written out and labelled here: union-find of the tile's 0-pixels.
+LV_D void ccl_local_body(const View& v, uint32_t bid, uint32_t nblk) {
+    constexpr int RW = CT + 4, DW = CT + 2;
+    __shared__ uint32_t lab[CT * CT];
+    __shared__ uint8_t s_raw[RW][RW];
+    __shared__ uint8_t s_dil[DW][DW];
+    const int ntx = (v.W + CT - 1) / CT;
+    const int x0 = ((int)bid % ntx) * CT, y0 = ((int)bid / ntx) * CT;
+    for (int i = threadIdx.x; i < RW * RW; i += TPB) {
+        const int ly = i / RW, lx = i % RW, gx = x0 + lx - 2, gy = y0 + ly - 2;
         s_raw[ly][lx] = (gx >= 0 && gx < v.W && gy >= 0 && gy < v.H) ? v.raw[(size_t)gy * v.W + gx] : 0;
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < (LW - 2) * (LH - 2); i += TPB) {
-        const int ly = i / (LW - 2), lx = i % (LW - 2), gx = x0 + lx - 1, gy = y0 + ly - 1;
-        uint8_t m = 0;
+    for (int i = threadIdx.x; i < DW * DW; i += TPB) {
+        const int ly = i / DW, lx = i % DW, gx = x0 + lx - 1, gy = y0 + ly - 1;
+        uint8_t m = 255;  // outside the image: neutral for the erosion
         if (gx >= 0 && gx < v.W && gy >= 0 && gy < v.H) {
+            m = 0;
             for (int a = 0; a < 3; ++a)
                 for (int b = 0; b < 3; ++b) m = s_raw[ly + a][lx + b] > m ? s_raw[ly + a][lx + b] : m;
-        } else {
-            m = 255;  // outside the image: neutral for the erosion
         }
         s_dil[ly][lx] = m;
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < PW * PH; i += TPB) {
-        const int ly = i / PW, lx = i % PW, gx = x0 + lx, gy = y0 + ly;
-        if (gx >= v.W || gy >= v.H) continue;
-        uint8_t m = 255;
-        for (int a = 0; a < 3; ++a)
-            for (int b = 0; b < 3; ++b) m = s_dil[ly + a][lx + b] < m ? s_dil[ly + a][lx + b] : m;
-        v.pre[(size_t)gy * v.W + gx] = m;
-    }
-}
-
-// --- hole fill: union-find labelling of the 0-pixels of PreFlood (4-connectivity) ---
-constexpr int CT = 32;  // CCL tile (32x32)
-
-LV_D void ccl_local_body(const View& v, uint32_t bid, uint32_t nblk) {
-    __shared__ uint32_t lab[CT * CT];
-    const int ntx = (v.W + CT - 1) / CT;
-    const int x0 = ((int)bid % ntx) * CT, y0 = ((int)bid / ntx) * CT;
     int any_fg = 0;
     for (int i = threadIdx.x; i < CT * CT; i += TPB) {
-        const int gx = x0 + i % CT, gy = y0 + i / CT;
-        const bool bgpx = gx < v.W && gy < v.H && v.pre[(size_t)gy * v.W + gx] == 0;
+        const int ly = i / CT, lx = i % CT, gx = x0 + lx, gy = y0 + ly;
+        bool bgpx = false;
+        if (gx < v.W && gy < v.H) {
+            uint8_t m = 255;
+            for (int a = 0; a < 3; ++a)
+                for (int b = 0; b < 3; ++b) m = s_dil[ly + a][lx + b] < m ? s_dil[ly + a][lx + b] : m;
+            v.pre[(size_t)gy * v.W + gx] = m;
+            bgpx = m == 0;
+        }
         lab[i] = bgpx ? (uint32_t)i : 0xFFFFFFFFu;
         any_fg |= bgpx ? 0 : 1;
     }
@@ -1386,16 +1379,28 @@ LV_D void ccl_border_body(const View& v, uint32_t bid, uint32_t nblk) {
 
 // holes = 0-pixels of PreFlood not connected to pixel (0,0) (floodFill + bitwise_not, SUB:541-543);
 // mh = raw | holes
+// One 32x32 tile per block. A background tile's pixels all carry the tile origin as their label,
+// so the origin's root is found once and shared.
 LV_D void holes_body(const View& v, uint32_t bid, uint32_t nblk) {
-    const uint32_t p = bid * TPB + threadIdx.x;
-    if (p >= v.npx) return;
-    uint8_t hole = 0;
-    if (v.pre[p] == 0) {
-        const uint32_t r = g_find(v.label, p);
-        v.label[p] = r;
-        hole = r != 0u ? 255 : 0;
+    __shared__ uint32_t s_root;
+    const int ntx = (v.W + CT - 1) / CT;
+    const int x0 = ((int)bid % ntx) * CT, y0 = ((int)bid / ntx) * CT;
+    const uint32_t origin = (uint32_t)(y0 * v.W + x0);
+    if (threadIdx.x == 0) s_root = v.pre[origin] == 0 ? g_find(v.label, origin) : 0xFFFFFFFFu;
+    __syncthreads();
+    const uint32_t root0 = s_root;
+    for (int i = threadIdx.x; i < CT * CT; i += TPB) {
+        const int gx = x0 + i % CT, gy = y0 + i / CT;
+        if (gx >= v.W || gy >= v.H) continue;
+        const uint32_t p = (uint32_t)(gy * v.W + gx);
+        uint8_t hole = 0;
+        if (v.pre[p] == 0) {
+            const uint32_t l = v.label[p];
+            const uint32_t r = (l == origin && root0 != 0xFFFFFFFFu) ? root0 : g_find(v.label, l);
+            hole = r != 0u ? 255 : 0;
+        }
+        v.mh[p] = v.raw[p] | hole;
     }
-    v.mh[p] = v.raw[p] | hole;
 }
 
 // mask = raw | holes | erode7(PreFlood); last_fg = medianBlur(mask, k) (SUB:544-547; LOB:578 uses
@@ -1517,20 +1522,62 @@ LV_D uint8_t sat_u8(float f) {
     return (uint8_t)(r < 0.f ? 0.f : r > 255.f ? 255.f : r);
 }
 
-// resize(INTER_AREA, W/8 x H/8) + accumulateWeighted LT/ST + per-cell |ST-LT| (SUB:568-583)
+LV_D bool ds_exact8(const View& v) { return v.W == 8 * v.dsW && v.H == 8 * v.dsH; }
+
+template <int C>
+LV_D void ds_update_cell(const View& v, uint32_t cell, const uint8_t m[C], uint32_t& contrib) {
+    const uint32_t frame_no = cur_frame(v), mvn = v.mv_avgs;
+    const float aL = 1.0f / (float)(frame_no < mvn ? frame_no : mvn);
+    const float aS = 1.0f / (float)(frame_no < mvn / 4u ? frame_no : mvn / 4u);
+    const float bL = 1.0f - aL, bS = 1.0f - aS;
+    uint32_t best = 0;
+    for (int c = 0; c < C; ++c) {
+        const size_t di = (size_t)cell * C + c;
+        const float lt = (float)m[c] * aL + v.dsLT[di] * bL;
+        const float st = (float)m[c] * aS + v.dsST[di] * bS;
+        v.dsLT[di] = lt;
+        v.dsST[di] = st;
+        const uint32_t d = (uint32_t)fabsf(st - lt);
+        best = d > best ? d : best;
+    }
+    contrib = C == 1 ? best / 2u : best;
+}
+
+// resize(INTER_AREA, W/8 x H/8) + accumulateWeighted LT/ST + per-cell |ST-LT| (SUB:568-583).
+// Frame sizes that are multiples of 8: one thread per pixel column of a row of 32 cells, 8 rows
+// summed per thread, the 8 columns of a cell reduced across lanes; other sizes: one thread per cell.
 template <int C>
 LV_D void downsample_body(const View& v, uint32_t bid, uint32_t nblk) {
     const uint8_t* frame = v.frame;
+    uint32_t contrib = 0;
+    if (ds_exact8(v)) {
+        const int nbx = (v.dsW + 31) / 32;
+        const int cy = (int)bid / nbx, cx = ((int)bid % nbx) * 32 + (int)threadIdx.x / 8;
+        const bool ok = cy < v.dsH && cx < v.dsW;
+        uint32_t sum[C];
+        for (int c = 0; c < C; ++c) sum[c] = 0;
+        if (ok) {
+            const int x = cx * 8 + (int)(threadIdx.x & 7u);
+            for (int r = 0; r < 8; ++r) {
+                const uint8_t* px = frame + ((size_t)(cy * 8 + r) * v.W + x) * C;
+                for (int c = 0; c < C; ++c) sum[c] += px[c];
+            }
+        }
+        for (int c = 0; c < C; ++c)
+            for (int o = 1; o < 8; o <<= 1) sum[c] += (uint32_t)__shfl_xor((int)sum[c], o, 64);
+        if (ok && (threadIdx.x & 7u) == 0) {
+            uint8_t m[C];
+            for (int c = 0; c < C; ++c) m[c] = sat_u8((float)sum[c] * (1.f / 64.f));
+            ds_update_cell<C>(v, (uint32_t)(cy * v.dsW + cx), m, contrib);
+        }
+        block_add<TPB>(contrib, v.sc->cd_part);
+        return;
+    }
     const uint32_t cell = bid * TPB + threadIdx.x;
     const uint32_t ncell = (uint32_t)(v.dsW * v.dsH);
-    uint32_t contrib = 0;
     if (cell < ncell) {
         const int dx = (int)(cell % (uint32_t)v.dsW), dy = (int)(cell / (uint32_t)v.dsW);
-        const uint32_t frame_no = cur_frame(v), mvn = v.mv_avgs;
-        const float aL = 1.0f / (float)(frame_no < mvn ? frame_no : mvn);
-        const float aS = 1.0f / (float)(frame_no < mvn / 4u ? frame_no : mvn / 4u);
-        const float bL = 1.0f - aL, bS = 1.0f - aS;
-        uint32_t best = 0;
+        uint8_t mc[C];
         for (int c = 0; c < C; ++c) {
             uint8_t m;
             if (v.W % v.dsW == 0 && v.H % v.dsH == 0) {
@@ -1552,15 +1599,9 @@ LV_D void downsample_body(const View& v, uint32_t bid, uint32_t nblk) {
                 }
                 m = sat_u8((float)(acc / (sx * sy)));
             }
-            const size_t di = (size_t)cell * C + c;
-            const float lt = (float)m * aL + v.dsLT[di] * bL;
-            const float st = (float)m * aS + v.dsST[di] * bS;
-            v.dsLT[di] = lt;
-            v.dsST[di] = st;
-            const uint32_t d = (uint32_t)fabsf(st - lt);
-            best = d > best ? d : best;
+            mc[c] = m;
         }
-        contrib = C == 1 ? best / 2u : best;
+        ds_update_cell<C>(v, cell, mc, contrib);
     }
     block_add<TPB>(contrib, v.sc->cd_part);
 }
@@ -1829,7 +1870,6 @@ template <int C, int ALGO, bool FUSED>
 __global__ void __launch_bounds__(TPB) k_update(View v) { update_body<C, ALGO, FUSED>(v, LV_SINGLE_BID); }
 template <int C>
 __global__ void __launch_bounds__(TPB) k_copy_last_color(View v) { copy_last_color_body<C>(v, LV_SINGLE_BID); }
-__global__ void __launch_bounds__(TPB) k_close(View v) { close_body(v, LV_SINGLE_BID); }
 __global__ void __launch_bounds__(TPB) k_ccl_local(View v) { ccl_local_body(v, LV_SINGLE_BID); }
 __global__ void __launch_bounds__(TPB) k_ccl_border(View v) { ccl_border_body(v, LV_SINGLE_BID); }
 __global__ void __launch_bounds__(TPB) k_holes(View v) { holes_body(v, LV_SINGLE_BID); }
@@ -1870,7 +1910,6 @@ __global__ void __launch_bounds__(256) kb_fixpoint_tail(BatchArgs b, uint32_t k0
     });
 }
 __global__ void __launch_bounds__(TPB) kb_update(BatchArgs b) { LV_BATCH_C(update_body, kSuBSENSE, true); }
-__global__ void __launch_bounds__(TPB) kb_close(BatchArgs b) { for_batch_block(b, close_body); }
 __global__ void __launch_bounds__(TPB) kb_ccl_local(BatchArgs b) { for_batch_block(b, ccl_local_body); }
 __global__ void __launch_bounds__(TPB) kb_ccl_border(BatchArgs b) { for_batch_block(b, ccl_border_body); }
 __global__ void __launch_bounds__(TPB) kb_holes(BatchArgs b) { for_batch_block(b, holes_body); }
@@ -1897,7 +1936,9 @@ void kernel_grids(const View& v, const StepOptions& opt, uint32_t g[kNumGrids]) 
     g[kGridCcl] = ct;
     g[kGridCclBorder] = cdiv((size_t)ct * 2 * CT, TPB);
     g[kGridPix] = cdiv(v.npx, TPB);
-    g[kGridCell] = v.lr_scaling ? cdiv((size_t)v.dsW * v.dsH, TPB) : 0u;
+    g[kGridCell] = !v.lr_scaling ? 0u
+                   : (v.W == 8 * v.dsW && v.H == 8 * v.dsH) ? cdiv(v.dsW, 32) * (unsigned)v.dsH
+                                                               : cdiv((size_t)v.dsW * v.dsH, TPB);
     g[kGridOne] = 1u;
     g[kGridFix] = (uint32_t)opt.fixpoint_blocks;
     if (opt.fix_px_per_block > 0) g[kGridFix] = std::min(g[kGridFix], cdiv(v.npx, (size_t)opt.fix_px_per_block));
@@ -1925,10 +1966,9 @@ static void frame_step_t(const View& v, const StepOptions& opt, hipStream_t st) 
                        (uint32_t)opt.tail_cap);
     if constexpr (ALGO == kSuBSENSE) {
         hipLaunchKernelGGL((k_update<C, ALGO, true>), dim3(g[kGridTile]), dim3(TPB), 0, st, v);
-        hipLaunchKernelGGL(k_close, dim3(g[kGridPost]), dim3(TPB), 0, st, v);
         hipLaunchKernelGGL(k_ccl_local, dim3(g[kGridCcl]), dim3(TPB), 0, st, v);
         hipLaunchKernelGGL(k_ccl_border, dim3(g[kGridCclBorder]), dim3(TPB), 0, st, v);
-        hipLaunchKernelGGL(k_holes, dim3(g[kGridPix]), dim3(TPB), 0, st, v);
+        hipLaunchKernelGGL(k_holes, dim3(g[kGridCcl]), dim3(TPB), 0, st, v);
         hipLaunchKernelGGL((k_median<true>), dim3(g[kGridPost]), dim3(TPB), 0, st, v);
         hipLaunchKernelGGL(k_dilate_final, dim3(g[kGridPost]), dim3(TPB), 0, st, v);
         if (g[kGridCell]) hipLaunchKernelGGL((k_downsample<C>), dim3(g[kGridCell]), dim3(TPB), 0, st, v);
@@ -1972,10 +2012,9 @@ void batch_step(const View* d_views, int ns, const uint32_t* const prefix[kNumGr
     hipLaunchKernelGGL(kb_fixpoint_tail, dim3(total[kGridOne]), dim3(256), 0, st, args(kGridOne),
                        (uint32_t)opt.fixpoint_launches + 1u, (uint32_t)opt.tail_cap);
     hipLaunchKernelGGL(kb_update, dim3(total[kGridTile]), dim3(TPB), 0, st, args(kGridTile));
-    hipLaunchKernelGGL(kb_close, dim3(total[kGridPost]), dim3(TPB), 0, st, args(kGridPost));
     hipLaunchKernelGGL(kb_ccl_local, dim3(total[kGridCcl]), dim3(TPB), 0, st, args(kGridCcl));
     hipLaunchKernelGGL(kb_ccl_border, dim3(total[kGridCclBorder]), dim3(TPB), 0, st, args(kGridCclBorder));
-    hipLaunchKernelGGL(kb_holes, dim3(total[kGridPix]), dim3(TPB), 0, st, args(kGridPix));
+    hipLaunchKernelGGL(kb_holes, dim3(total[kGridCcl]), dim3(TPB), 0, st, args(kGridCcl));
     hipLaunchKernelGGL(kb_median, dim3(total[kGridPost]), dim3(TPB), 0, st, args(kGridPost));
     hipLaunchKernelGGL(kb_dilate_final, dim3(total[kGridPost]), dim3(TPB), 0, st, args(kGridPost));
     if (total[kGridCell]) hipLaunchKernelGGL(kb_downsample, dim3(total[kGridCell]), dim3(TPB), 0, st, args(kGridCell));
