@@ -220,7 +220,9 @@ int lvbgs_server_create(int device, const lvbgs_server_options* opts, lvbgs_serv
 void lvbgs_server_destroy(lvbgs_server* s);
 const char* lvbgs_server_last_error(const lvbgs_server* s);
 void* lvbgs_server_stream(lvbgs_server* s);
-/* admits a SuBSENSE stream initialized from d_img0 (blocking); LVBGS_ERR_BUDGET when it does not fit */
+/* admits a SuBSENSE stream initialized from d_img0 (full-frame ROI); LVBGS_ERR_BUDGET when it does not
+ * fit. The initialization is queued on the device: d_img0 must stay valid until the stream's first
+ * frame completes; its frames join steps once the initialization is done. */
 int lvbgs_server_open(lvbgs_server* s, const lvbgs_params* params, const void* d_img0, int width, int height,
                       int channels, int* stream_out);
 /* no further frames; the stream's waiting frames still complete, then its id may be reused */

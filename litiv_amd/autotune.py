@@ -15,8 +15,8 @@ import numpy as np
 CANDIDATES = [
     {"fix_px_per_block": 8192, "fixpoint_launches": 3, "seq_limit": 255, "fetch_depth": 2},
     {"fix_px_per_block": 8192, "fixpoint_launches": 3, "seq_limit": 255, "fetch_depth": 4},
-    {"fix_px_per_block": 32768, "fixpoint_launches": 3, "seq_limit": 255, "fetch_depth": 2},
-    {"fix_px_per_block": 8192, "fixpoint_launches": 2, "seq_limit": 255, "fetch_depth": 2},
+    {"fix_px_per_block": 8192, "fixpoint_launches": 2, "seq_limit": 255, "fetch_depth": 4},
+    {"fix_px_per_block": 8192, "fixpoint_launches": 1, "seq_limit": 255, "fetch_depth": 4},
     {"fix_px_per_block": 8192, "fixpoint_launches": 4, "seq_limit": 255, "fetch_depth": 2},
     {"fix_px_per_block": 8192, "fixpoint_launches": 3, "seq_limit": 12, "cont_px_per_block": 2048, "fetch_depth": 2},
 ]

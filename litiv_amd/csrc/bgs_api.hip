@@ -217,22 +217,29 @@ static int download_scalars(lvbgs_ctx* c, Scalars& s) {
     return hip_check(c, hipStreamSynchronize(c->stream), "sync");
 }
 
+// done == nullptr: synchronous (the reference's contract); otherwise the work stays queued on the
+// context's stream and `done` is recorded after it
 static int initialize(lvbgs_ctx* c, const uint8_t* img, bool device_img, int w, int h, int ch, size_t stride,
-                      const uint8_t* roi, size_t roi_stride);
+                      const uint8_t* roi, size_t roi_stride, hipEvent_t done);
 
 extern "C" int lvbgs_initialize(lvbgs_ctx* c, const uint8_t* img, int w, int h, int ch, size_t stride,
                                 const uint8_t* roi, size_t roi_stride) {
-    return initialize(c, img, false, w, h, ch, stride, roi, roi_stride);
+    return initialize(c, img, false, w, h, ch, stride, roi, roi_stride, nullptr);
 }
 
 extern "C" int lvbgs_initialize_device(lvbgs_ctx* c, const void* d_img, int w, int h, int ch, const uint8_t* roi,
                                        size_t roi_stride) {
     return initialize(c, (const uint8_t*)d_img, true, w, h, ch, (size_t)w * (size_t)(ch > 0 ? ch : 1), roi,
-                      roi_stride);
+                      roi_stride, nullptr);
+}
+
+int lvbgs_initialize_async(lvbgs_ctx* c, const void* d_img, int w, int h, int ch, hipEvent_t done) {
+    return initialize(c, (const uint8_t*)d_img, true, w, h, ch, (size_t)w * (size_t)(ch > 0 ? ch : 1), nullptr, 0,
+                      done);
 }
 
 static int initialize(lvbgs_ctx* c, const uint8_t* img, bool device_img, int w, int h, int ch, size_t stride,
-                      const uint8_t* roi, size_t roi_stride) {
+                      const uint8_t* roi, size_t roi_stride, hipEvent_t done) {
     if (!c) return LVBGS_ERR_ARG;
     c->err.clear();
     if (!img || w <= 0 || h <= 0) return set_err(c, LVBGS_ERR_ARG, "provided image for initialization must be non-empty");
@@ -243,11 +250,15 @@ static int initialize(lvbgs_ctx* c, const uint8_t* img, bool device_img, int w, 
     if ((size_t)w * (size_t)h > 0x7FFFFFFFull / 8) return set_err(c, LVBGS_ERR_ARG, "image too large");
     if (hipSetDevice(c->device) != hipSuccess) return set_err(c, LVBGS_ERR_HIP, "hipSetDevice");
     const size_t npx = (size_t)w * h;
-    // ROI (BackgroundSubtractionUtils.cpp:82-101)
-    std::vector<uint8_t> nr(npx);
-    if (!roi) {
-        if (!c->roi.empty() && c->roi_w == w && c->roi_h == h) nr = c->roi;  // reuse (:87-88)
-        else std::memset(nr.data(), 255, npx);
+    // ROI (BackgroundSubtractionUtils.cpp:82-101); the last ROI of the same size is reused as is
+    // (:87-88), with the counts it had
+    const bool reuse_roi = !roi && !c->roi.empty() && c->roi_w == w && c->roi_h == h;
+    std::vector<uint8_t> nr;
+    size_t orig = c->orig_roi, fin = c->n_roi;
+    if (!reuse_roi) nr.resize(npx);
+    if (reuse_roi) {
+    } else if (!roi) {
+        std::memset(nr.data(), 255, npx);
     } else {
         if (roi_stride == 0) roi_stride = (size_t)w;
         if (roi_stride < (size_t)w) return set_err(c, LVBGS_ERR_ARG, "ROI stride too small");
@@ -273,13 +284,15 @@ static int initialize(lvbgs_ctx* c, const uint8_t* img, bool device_img, int w, 
             }
         for (size_t i = 0; i < npx; ++i) nr[i] = nr[i] | dil[i];
     }
-    size_t orig = 0;
-    for (size_t i = 0; i < npx; ++i) orig += nr[i] ? 1 : 0;
-    if (!orig) return set_err(c, LVBGS_ERR_ARG, "provided ROI mat contains no useful pixels");
-    lvbgs_validate_roi(w, h, nr.data(), (size_t)w);
-    size_t fin = 0;
-    for (size_t i = 0; i < npx; ++i) fin += nr[i] ? 1 : 0;
-    if (!fin) return set_err(c, LVBGS_ERR_ARG, "provided ROI mat contains no useful pixels away from borders");
+    if (!reuse_roi) {
+        orig = 0;
+        for (size_t i = 0; i < npx; ++i) orig += nr[i] ? 1 : 0;
+        if (!orig) return set_err(c, LVBGS_ERR_ARG, "provided ROI mat contains no useful pixels");
+        lvbgs_validate_roi(w, h, nr.data(), (size_t)w);
+        fin = 0;
+        for (size_t i = 0; i < npx; ++i) fin += nr[i] ? 1 : 0;
+        if (!fin) return set_err(c, LVBGS_ERR_ARG, "provided ROI mat contains no useful pixels away from borders");
+    }
 
     c->initialized = false;
     c->W = w;
@@ -288,7 +301,7 @@ static int initialize(lvbgs_ctx* c, const uint8_t* img, bool device_img, int w, 
     c->npx = npx;
     c->orig_roi = orig;
     c->n_roi = fin;
-    c->roi = nr;
+    if (!reuse_roi) c->roi = std::move(nr);
     c->roi_w = w;
     c->roi_h = h;
     c->apply_count = 0;
@@ -347,7 +360,8 @@ static int initialize(lvbgs_ctx* c, const uint8_t* img, bool device_img, int w, 
     s.lower_cap = lower;
     s.upper_cap = upper;
     if ((rc = upload_scalars(c, s))) return rc;
-    if ((rc = hip_check(c, hipMemcpyAsync((void*)v.roi, nr.data(), npx, hipMemcpyHostToDevice, c->stream), "roi"))) return rc;
+    if ((rc = hip_check(c, hipMemcpyAsync((void*)v.roi, c->roi.data(), npx, hipMemcpyHostToDevice, c->stream), "roi")))
+        return rc;
     const uint8_t* d_init = img;
     if (!device_img) {
         if ((rc = hip_check(c, hipMemcpy2DAsync(c->d_frame, (size_t)w * ch, img, stride, (size_t)w * ch, (size_t)h,
@@ -370,7 +384,11 @@ static int initialize(lvbgs_ctx* c, const uint8_t* img, bool device_img, int w, 
                                           c->stream), "scalars")))
         return rc;
     if ((rc = hip_check(c, hipGetLastError(), "initialize launch"))) return rc;
-    if ((rc = hip_check(c, hipStreamSynchronize(c->stream), "initialize"))) return rc;
+    if (done) {
+        if ((rc = hip_check(c, hipEventRecord(done, c->stream), "initialize event"))) return rc;
+    } else if ((rc = hip_check(c, hipStreamSynchronize(c->stream), "initialize"))) {
+        return rc;
+    }
     c->initialized = true;
     return LVBGS_OK;
 }

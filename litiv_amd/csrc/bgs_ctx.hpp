@@ -38,3 +38,7 @@ struct lvbgs_ctx {
 
 // device bytes of the model and per-step buffers of a stream of this geometry (one arena)
 size_t lvbgs_arena_bytes(int algo, const lvbgs_params& p, int width, int height, int channels);
+
+// initialize from a device frame with the previous full-frame ROI of the same size (or a full frame),
+// left queued on the context's stream; `done` is recorded after it
+int lvbgs_initialize_async(lvbgs_ctx* c, const void* d_img, int width, int height, int channels, hipEvent_t done);

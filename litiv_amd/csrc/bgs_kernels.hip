@@ -846,7 +846,7 @@ LV_D void match_cont_body(const View& v, uint32_t bid, uint32_t nblk) {
     __shared__ uint8_t lut[256];
     __shared__ uint8_t patches[TPB / kGroup][25 * C];
     const uint32_t n = v.sc->cont_count;
-    if (n == 0) return;  // uniform: nothing was left to this phase
+    if (bid * (TPB / kGroup) >= n) return;  // uniform: no item reaches this block
     load_lut(lut, v.sc);
     __syncthreads();
     uint8_t* patch = patches[threadIdx.x / kGroup];
@@ -938,6 +938,7 @@ LV_D void fixpoint_body(const View& v, uint32_t bid, uint32_t nblk, uint32_t k) 
         atomicMax(&v.sc->max_iter_seen, k);
         if (k < 8) v.sc->wl_work[k] = n;
     }
+    if (bid * (TPB / kGroup) >= n) return;  // uniform: no item reaches this block
     load_lut(lut, v.sc);
     __syncthreads();
     const uint32_t* wl = (k & 1u) ? v.wl1 : v.wl0;
@@ -1418,48 +1419,67 @@ LV_D void median_body(const View& v, uint32_t bid, uint32_t nblk) {
     const int tid = (int)bid;
     const int x0 = (tid % ntx) * PW, y0 = (tid / ntx) * PH;
     const int k = v.median_k, r = k / 2, need = (k * k + 1) / 2;
+    // every window pass below slides its window along a run of outputs (one add, one remove)
     if constexpr (SUBSENSE) {
         for (int i = threadIdx.x; i < EW * EH; i += TPB) {
             const int ly = i / EW, lx = i % EW, gx = x0 + lx - HM - HE, gy = y0 + ly - HM - HE;
             s_pre[ly][lx] = (gx >= 0 && gx < v.W && gy >= 0 && gy < v.H) ? v.pre[(size_t)gy * v.W + gx] : 255;
         }
         __syncthreads();
-        for (int i = threadIdx.x; i < EH * MW; i += TPB) {
-            const int ly = i / MW, lx = i % MW;
-            uint8_t m = 255;
-            for (int b = 0; b < 2 * HE + 1; ++b) m = s_pre[ly][lx + b] < m ? s_pre[ly][lx + b] : m;
-            s_eh[ly][lx] = m;
+        // horizontal 7-erosion of the 0/255 PreFlood: 0 when the window holds a 0
+        constexpr int HS = 4, HL = MW / HS;  // 4 runs of 19 per row
+        for (int it = threadIdx.x; it < EH * HS; it += TPB) {
+            const int ly = it / HS, xs = (it % HS) * HL;
+            int z = 0;
+            for (int b = 0; b < 2 * HE + 1; ++b) z += s_pre[ly][xs + b] == 0;
+            for (int lx = xs; lx < xs + HL; ++lx) {
+                s_eh[ly][lx] = z ? 0 : 255;
+                if (lx + 1 < xs + HL) z += (int)(s_pre[ly][lx + 2 * HE + 1] == 0) - (int)(s_pre[ly][lx] == 0);
+            }
         }
         __syncthreads();
     }
-    for (int i = threadIdx.x; i < MW * MH; i += TPB) {
-        const int ly = i / MW, lx = i % MW, gx = x0 + lx - HM, gy = y0 + ly - HM;
-        uint8_t m = 0;
-        if (gx >= 0 && gx < v.W && gy >= 0 && gy < v.H) {
-            if constexpr (SUBSENSE) {
-                uint8_t e = 255;
-                for (int a = 0; a < 2 * HE + 1; ++a) e = s_eh[ly + a][lx] < e ? s_eh[ly + a][lx] : e;
-                m = v.mh[(size_t)gy * v.W + gx] | e;
-            } else {
-                m = v.raw[(size_t)gy * v.W + gx];
+    // mask = raw | holes | erode7(PreFlood) (LOBSTER: raw), as 0/1, with the vertical erosion
+    constexpr int VS = 2, VL = MH / VS;  // 2 runs of 14 per column
+    for (int it = threadIdx.x; it < MW * VS; it += TPB) {
+        const int lx = it % MW, ys = (it / MW) * VL;
+        int z = 0;
+        if constexpr (SUBSENSE)
+            for (int a = 0; a < 2 * HE + 1; ++a) z += s_eh[ys + a][lx] == 0;
+        for (int ly = ys; ly < ys + VL; ++ly) {
+            const int gx = x0 + lx - HM, gy = y0 + ly - HM;
+            uint8_t m = 0;
+            if (gx >= 0 && gx < v.W && gy >= 0 && gy < v.H) {
+                if constexpr (SUBSENSE) m = v.mh[(size_t)gy * v.W + gx] | (z ? 0 : 255);
+                else m = v.raw[(size_t)gy * v.W + gx];
             }
+            s_mask[ly][lx] = m ? 1 : 0;
+            if constexpr (SUBSENSE)
+                if (ly + 1 < ys + VL) z += (int)(s_eh[ly + 2 * HE + 1][lx] == 0) - (int)(s_eh[ly][lx] == 0);
         }
-        s_mask[ly][lx] = m ? 1 : 0;
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < PH * MW; i += TPB) {
-        const int ly = i / MW, lx = i % MW;
+    // column counts over k rows, one run of PH outputs per column
+    for (int lx = threadIdx.x; lx < MW; lx += TPB) {
         int c = 0;
-        for (int a = -r; a <= r; ++a) c += s_mask[ly + HM + a][lx];
-        s_col[ly][lx] = (uint8_t)c;
+        for (int a = -r; a <= r; ++a) c += s_mask[HM + a][lx];
+        for (int ly = 0; ly < PH; ++ly) {
+            s_col[ly][lx] = (uint8_t)c;
+            if (ly + 1 < PH) c += (int)s_mask[HM + ly + 1 + r][lx] - (int)s_mask[HM + ly - r][lx];
+        }
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < PW * PH; i += TPB) {
-        const int ly = i / PW, lx = i % PW, gx = x0 + lx, gy = y0 + ly;
-        if (gx >= v.W || gy >= v.H) continue;
+    // row counts over k columns and the majority decision, runs of 16 outputs
+    constexpr int RS = PW / 16;
+    for (int it = threadIdx.x; it < PH * RS; it += TPB) {
+        const int ly = it / RS, xs = (it % RS) * 16, gy = y0 + ly;
         int c = 0;
-        for (int b = -r; b <= r; ++b) c += s_col[ly][lx + HM + b];
-        v.last_fg[(size_t)gy * v.W + gx] = c >= need ? 255 : 0;
+        for (int b = -r; b <= r; ++b) c += s_col[ly][xs + HM + b];
+        for (int lx = xs; lx < xs + 16; ++lx) {
+            const int gx = x0 + lx;
+            if (gx < v.W && gy < v.H) v.last_fg[(size_t)gy * v.W + gx] = c >= need ? 255 : 0;
+            if (lx + 1 < xs + 16) c += (int)s_col[ly][lx + 1 + HM + r] - (int)s_col[ly][lx + HM - r];
+        }
     }
 }
 

@@ -36,6 +36,7 @@ struct Slot {
     int inflight = 0;  // frames of this stream in queued steps
     bool open = false;
     bool closing = false;
+    bool ready = false;  // its initialization (queued on the context's stream) has completed
 };
 
 struct QueuedStep {
@@ -53,6 +54,7 @@ struct lvbgs_server {
     StepOptions opt{};
     size_t used = 0;  // arena bytes of open and pooled contexts
     std::vector<Slot> slots;
+    std::vector<hipEvent_t> init_ev;  // per slot id: end of the stream's initialization
     std::vector<int> free_ids;
     std::vector<lvbgs_ctx*> pool;  // closed contexts whose arenas are kept for reuse
     View* d_views = nullptr;
@@ -98,6 +100,7 @@ static void server_release(lvbgs_server* s) {
     for (auto e : s->stage_ev) (void)hipEventDestroy(e);
     for (auto e : s->ev_free) (void)hipEventDestroy(e);
     for (auto& q : s->queued) (void)hipEventDestroy(q.done);
+    for (auto e : s->init_ev) (void)hipEventDestroy(e);
     if (s->stream) (void)hipStreamDestroy(s->stream);
 }
 
@@ -194,7 +197,15 @@ static int launch_step(lvbgs_server* s) {
     std::vector<int> members;
     for (int k = 0; k < nslots && (int)members.size() < s->o.max_batch; ++k) {
         const int i = (s->rr + k) % nslots;
-        if (s->slots[(size_t)i].open && !s->slots[(size_t)i].pending.empty()) members.push_back(i);
+        Slot& sl = s->slots[(size_t)i];
+        if (!sl.open || sl.pending.empty()) continue;
+        if (!sl.ready) {
+            const hipError_t e = hipEventQuery(s->init_ev[(size_t)i]);
+            if (e == hipErrorNotReady) continue;
+            if (e != hipSuccess) return srv_err(s, LVBGS_ERR_HIP, "stream initialization failed");
+            sl.ready = true;
+        }
+        members.push_back(i);
     }
     if (members.empty()) return LVBGS_OK;
     s->rr = (members.back() + 1) % nslots;
@@ -324,23 +335,33 @@ extern "C" int lvbgs_server_open(lvbgs_server* s, const lvbgs_params* params, co
         c->p.rng_seed = p.rng_seed;
         c->p.stream_id = p.stream_id;
     }
-    const size_t before = c->arena_size;
-    const int rc = lvbgs_initialize_device(c, d_img0, width, height, channels, nullptr, 0);
-    if (rc) {
-        s->err = lvbgs_last_error(c);
-        s->used -= before;
-        lvbgs_destroy(c);
-        return rc;
-    }
-    s->used += c->arena_size - before;
     int id;
     if (!s->free_ids.empty()) {
         id = s->free_ids.back();
         s->free_ids.pop_back();
     } else {
+        hipEvent_t e = nullptr;
+        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+            if (c->arena_size) s->pool.push_back(c);
+            else lvbgs_destroy(c);
+            return srv_err(s, LVBGS_ERR_HIP, "event");
+        }
         id = (int)s->slots.size();
         s->slots.emplace_back();
+        s->init_ev.push_back(e);
     }
+    // initialization is queued on the context's own stream (no host wait): the stream joins steps
+    // once its event has completed
+    const size_t before = c->arena_size;
+    const int rc = lvbgs_initialize_async(c, d_img0, width, height, channels, s->init_ev[(size_t)id]);
+    if (rc) {
+        s->err = lvbgs_last_error(c);
+        s->used -= before;
+        lvbgs_destroy(c);
+        s->free_ids.push_back(id);
+        return rc;
+    }
+    s->used += c->arena_size - before;
     Slot& sl = s->slots[(size_t)id];
     sl = Slot();
     sl.ctx = c;
