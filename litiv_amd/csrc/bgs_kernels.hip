@@ -157,8 +157,17 @@ LV_D void store_sample(const View& v, uint32_t s, uint32_t p, const uint32_t col
     uint64_t ra;
     uint32_t rb;
     encode_raw<C>(col, desc, ra, rb);
-    if constexpr (C == 3) ((Rec12*)v.sA)[idx] = Rec12{(uint32_t)ra, (uint32_t)(ra >> 32), rb};
-    else ((uint32_t*)v.sA)[idx] = (uint32_t)ra;
+    // non-temporal: scattered single-record writes gain nothing from being kept in L2
+    typedef uint32_t u32x3 __attribute__((ext_vector_type(3)));
+    if constexpr (C == 3) {
+        u32x3 w;
+        w.x = (uint32_t)ra;
+        w.y = (uint32_t)(ra >> 32);
+        w.z = rb;
+        __builtin_nontemporal_store(w, (u32x3*)((Rec12*)v.sA + idx));
+    } else {
+        __builtin_nontemporal_store((uint32_t)ra, (uint32_t*)v.sA + idx);
+    }
 }
 
 // current colour and intra descriptor of pixel q (the value it writes into samples)
@@ -773,11 +782,34 @@ LV_D void match_body(const View& v, uint32_t bid, uint32_t nblk) {
     const int ntx = (v.W + TW - 1) / TW, nty = (v.H + TH - 1) / TH;
     const int tid = (int)bid;
     const int x0 = (tid % ntx) * TW, y0 = (tid / ntx) * TH;
+    __shared__ uint16_t s_perm[TPB];
+    __shared__ uint32_t s_nfg[TPB / 64];
     load_frame_tile<C>(tile, frame, v.W, v.H, x0, y0);
     load_pred_tile(pred, v, x0, y0);
     load_lut(lut, v.sc);
     __syncthreads();
-    const int lx = threadIdx.x % TW, ly = threadIdx.x / TW, x = x0 + lx, y = y0 + ly;
+    // A wavefront scans as long as its slowest lane, and foreground pixels scan every sample: the
+    // tile's pixels are regrouped so those that were foreground in the previous frame share
+    // wavefronts (stable order within each group). Each pixel's match depends on the tile data only.
+    {
+        const int t = (int)threadIdx.x, ox = x0 + t % TW, oy = y0 + t / TW;
+        const bool likely_fg = ox < v.W && oy < v.H && pred.guess[t / TW + 2][t % TW + 2] != 0 &&
+                               v.roi[(size_t)oy * v.W + ox];
+        const unsigned long long bal = __ballot(likely_fg);
+        const int lane = t & 63, wave = t >> 6;
+        if (lane == 0) s_nfg[wave] = (uint32_t)__popcll(bal);
+        __syncthreads();
+        uint32_t fg_before = (uint32_t)__popcll(bal & ((1ull << lane) - 1ull)), nfg = 0;
+        for (int w = 0; w < TPB / 64; ++w) {
+            fg_before += w < wave ? s_nfg[w] : 0u;
+            nfg += s_nfg[w];
+        }
+        const uint32_t pos = likely_fg ? (uint32_t)TPB - nfg + fg_before : (uint32_t)t - fg_before;
+        s_perm[pos] = (uint16_t)t;
+        __syncthreads();
+    }
+    const int src = s_perm[threadIdx.x];
+    const int lx = src % TW, ly = src / TW, x = x0 + lx, y = y0 + ly;
     const uint32_t p = (uint32_t)(y * v.W + x);
     uint32_t examined = 0;
     bool cont = false;
