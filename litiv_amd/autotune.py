@@ -13,12 +13,12 @@ import numpy as np
 # fix_px_per_block / cont_px_per_block scale each stream's grid with its pixel count, so streams of
 # every resolution in the batch get a grid sized to their worklist
 CANDIDATES = [
-    {"fix_px_per_block": 8192, "fixpoint_launches": 3, "seq_limit": 255, "fetch_depth": 2},
-    {"fix_px_per_block": 8192, "fixpoint_launches": 3, "seq_limit": 255, "fetch_depth": 4},
     {"fix_px_per_block": 8192, "fixpoint_launches": 2, "seq_limit": 255, "fetch_depth": 4},
-    {"fix_px_per_block": 8192, "fixpoint_launches": 1, "seq_limit": 255, "fetch_depth": 4},
-    {"fix_px_per_block": 8192, "fixpoint_launches": 4, "seq_limit": 255, "fetch_depth": 2},
-    {"fix_px_per_block": 8192, "fixpoint_launches": 3, "seq_limit": 12, "cont_px_per_block": 2048, "fetch_depth": 2},
+    {"fix_px_per_block": 8192, "fixpoint_launches": 2, "seq_limit": 255, "fetch_depth": 2},
+    {"fix_px_per_block": 8192, "fixpoint_launches": 3, "seq_limit": 255, "fetch_depth": 4},
+    {"fix_px_per_block": 8192, "fixpoint_launches": 2, "seq_limit": 8, "cont_px_per_block": 2048, "fetch_depth": 4},
+    {"fix_px_per_block": 8192, "fixpoint_launches": 2, "seq_limit": 12, "cont_px_per_block": 2048, "fetch_depth": 4},
+    {"fix_px_per_block": 8192, "fixpoint_launches": 2, "seq_limit": 16, "cont_px_per_block": 2048, "fetch_depth": 4},
 ]
 
 

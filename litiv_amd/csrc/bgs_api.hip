@@ -159,6 +159,7 @@ static void carve(lvbgs_ctx* c, Arena& ar) {
     v.wl0 = ar.take<uint32_t>(npx);
     v.wl1 = ar.take<uint32_t>(npx);
     v.cont = ar.take<uint32_t>(npx);
+    v.cont2 = ar.take<uint32_t>(npx);
     v.stamp = ar.take<unsigned long long>(npx);
     v.dsLT = ar.take<float>(nds);
     v.dsST = ar.take<float>(nds);

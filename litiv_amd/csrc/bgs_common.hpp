@@ -155,7 +155,8 @@ struct alignas(16) Scalars {
     uint32_t max_iter_seen;      // largest fix-point iteration index that had work (sticky)
     uint32_t epoch;              // steps since allocation (worklist de-duplication tags)
     uint32_t wl_work[8];         // per-frame: worklist size of fix-point iterations 1..7
-    uint32_t cont_count;         // per-frame: pixels handed to the cooperative match phase
+    uint32_t cont_count;         // per-frame: pixels handed to the match continuation
+    uint32_t cont2_count;        // per-frame: pixels handed to the cooperative match phase
     uint8_t lut[256];            // LBSP threshold LUT (m_anLBSPThreshold_8bitLUT)
 };
 constexpr uint32_t kErrFixpointCap = 1u;

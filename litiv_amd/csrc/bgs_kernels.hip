@@ -616,6 +616,7 @@ LV_D void describe_body(const View& v, uint32_t bid, uint32_t nblk) {
         // per-frame counters consumed by later kernels of this step
         for (int i = 0; i < kWlRing; ++i) v.sc->wl_count[i] = 0;
         v.sc->cont_count = 0;
+        v.sc->cont2_count = 0;
         for (int i = 0; i < 8; ++i) v.sc->wl_work[i] = 0;
         v.sc->iters = 0;
     }
@@ -716,6 +717,31 @@ LV_D void build_ctx(const View& v, const uint8_t* frame, const volatile uint8_t*
             for (int k = 0; k < 4; ++k) {
                 const int n = 4 * w + k;
                 acc |= (uint32_t)win[(kLbspDY[n] + 2) * RS + (kLbspDX[n] + 2) * C + c] << (8 * k);
+            }
+            px.taps[c][w] = acc;
+        }
+    if constexpr (ALGO == kSuBSENSE) subsense_thresholds<C>(v, v.R[p], v.unstable[p], px);
+    else lobster_thresholds<C>(v, px);
+}
+
+// the same context with the taps read from the frame in global memory
+template <int C, int ALGO>
+LV_D void build_ctx_global(const View& v, const uint8_t* frame, int x, int y, PxCtx<C>& px) {
+    const uint32_t p = (uint32_t)(y * v.W + x);
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+        px.cur[c] = frame[(size_t)p * C + c];
+        px.intra[c] = v.desc_cur[(size_t)p * C + c];
+    }
+#pragma unroll
+    for (int c = 0; c < C; ++c)
+#pragma unroll
+        for (int w = 0; w < 4; ++w) {
+            uint32_t acc = 0;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const int n = 4 * w + k;
+                acc |= (uint32_t)frame[((size_t)(y + kLbspDY[n]) * v.W + (x + kLbspDX[n])) * C + c] << (8 * k);
             }
             px.taps[c][w] = acc;
         }
@@ -871,13 +897,60 @@ LV_D uint32_t applied_writers_global(const View& v, int x, int y) {
     return applied;
 }
 
-// Phase B: pixels left unfinished by phase A (mostly foreground), kGroup lanes per pixel
+// decision of a finished scan: match result, raw decision, and the forward target of a pixel whose
+// decision differs from the speculative guess goes to the first fix-point worklist
+template <int C>
+LV_D void finish_pixel(const View& v, uint32_t p, int x, int y, const ScanState& st, uint64_t epoch) {
+    v.mres[p] = pack_state(st);
+    const uint8_t fg = st.good < (uint32_t)v.REQ ? 1 : 0;
+    v.dec[p] = fg;
+    const uint8_t guess = v.last_raw[p] ? 1 : 0;
+    uint32_t t;
+    if (fg != guess && forward_target(v, v.rec[p], x, y, t)) wl_push(v, t, 1u, epoch, v.wl1);
+}
+
+// Phase B: the scans phase A cut at its sample limit, continued with one lane per pixel over the
+// dense list of unfinished pixels, so long scans (mostly foreground pixels, which examine every
+// sample) fill whole wavefronts. The context is rebuilt from global memory; a pixel with more than
+// kRepl overwritten slots goes on to phase C.
+template <int C, int ALGO, int KF>
+LV_D void match_seq_body(const View& v, uint32_t bid, uint32_t nblk) {
+    const uint8_t* frame = v.frame;
+    __shared__ uint8_t lut[256];
+    const uint32_t n = v.sc->cont_count;
+    if (bid * TPB >= n) return;  // uniform: no item reaches this block
+    load_lut(lut, v.sc);
+    __syncthreads();
+    const uint64_t epoch = v.sc->epoch;
+    uint32_t examined = 0;
+    for (uint32_t i = bid * TPB + threadIdx.x; i < n; i += nblk * TPB) {
+        const uint32_t p = v.cont[i];
+        const int x = (int)(p % (uint32_t)v.W), y = (int)(p / (uint32_t)v.W);
+        PxCtx<C> px;
+        build_ctx_global<C, ALGO>(v, frame, x, y, px);
+        const uint32_t applied = applied_writers_global<false>(v, x, y);
+        uint64_t sbm;
+        int nrep;
+        if (!build_repl<C>(v, frame, x, y, applied, sbm, nrep)) {
+            v.cont2[atomicAdd(&v.sc->cont2_count, 1u)] = p;
+            continue;
+        }
+        ScanState st = unpack_state(v.mres[p]);
+        const uint32_t s0 = st.s;
+        scan_seq<C, ALGO, 0, KF>(v, lut, px, x, y, sbm, nrep, st, (uint32_t)v.N);
+        examined += st.s - s0;
+        finish_pixel<C>(v, p, x, y, st, epoch);
+    }
+    block_add<TPB>(examined, v.sc->coop_part);
+}
+
+// Phase C: pixels left by phase B (more than kRepl overwritten slots), kGroup lanes per pixel
 template <int C, int ALGO>
 LV_D void match_cont_body(const View& v, uint32_t bid, uint32_t nblk) {
     const uint8_t* frame = v.frame;
     __shared__ uint8_t lut[256];
     __shared__ uint8_t patches[TPB / kGroup][25 * C];
-    const uint32_t n = v.sc->cont_count;
+    const uint32_t n = v.sc->cont2_count;
     if (bid * (TPB / kGroup) >= n) return;  // uniform: no item reaches this block
     load_lut(lut, v.sc);
     __syncthreads();
@@ -891,7 +964,7 @@ LV_D void match_cont_body(const View& v, uint32_t bid, uint32_t nblk) {
     for (uint32_t r = 0; r < rounds; ++r) {
         const uint32_t i = g0 + r * ngroups;
         const bool active = i < n;
-        const uint32_t p = active ? v.cont[i] : 0u;
+        const uint32_t p = active ? v.cont2[i] : 0u;
         const int x = (int)(p % (uint32_t)v.W), y = (int)(p / (uint32_t)v.W);
         PxCtx<C> px;
         uint32_t applied = 0;
@@ -910,12 +983,7 @@ LV_D void match_cont_body(const View& v, uint32_t bid, uint32_t nblk) {
         scan_coop<C, ALGO, 5 * C>(v, frame, lut, px, x, y, applied, sbm, st, active);
         if (active && sub == 0) {
             examined += st.s - s_before;
-            v.mres[p] = pack_state(st);
-            const uint8_t fg = st.good < (uint32_t)v.REQ ? 1 : 0;
-            v.dec[p] = fg;
-            const uint8_t guess = v.last_raw[p] ? 1 : 0;
-            uint32_t t;
-            if (fg != guess && forward_target(v, v.rec[p], x, y, t)) wl_push(v, t, 1u, epoch, v.wl1);
+            finish_pixel<C>(v, p, x, y, st, epoch);
         }
     }
     block_add<TPB>(examined, v.sc->coop_part);
@@ -1908,6 +1976,10 @@ __global__ void __launch_bounds__(TPB) k_describe(View v) { describe_body<C, ALG
 #define LV_MATCH_WAVES(KF) __attribute__((amdgpu_waves_per_eu(5)))
 template <int C, int ALGO, int KF>
 __global__ void __launch_bounds__(TPB) LV_MATCH_WAVES(KF) k_match(View v) { match_body<C, ALGO, KF>(v, LV_SINGLE_BID); }
+template <int C, int ALGO, int KF>
+__global__ void __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(4))) k_match_seq(View v) {
+    match_seq_body<C, ALGO, KF>(v, LV_SINGLE_BID);
+}
 template <int C, int ALGO>
 __global__ void __launch_bounds__(TPB) k_match_cont(View v) { match_cont_body<C, ALGO>(v, LV_SINGLE_BID); }
 template <int C, int ALGO>
@@ -1948,6 +2020,10 @@ __global__ void __launch_bounds__(TPB) k_refresh(View v, int use_pending, float 
 __global__ void __launch_bounds__(TPB) kb_describe(BatchArgs b) { LV_BATCH_C(describe_body, kSuBSENSE); }
 template <int KF>
 __global__ void __launch_bounds__(TPB) LV_MATCH_WAVES(KF) kb_match(BatchArgs b) { LV_BATCH_C(match_body, kSuBSENSE, KF); }
+template <int KF>
+__global__ void __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(4))) kb_match_seq(BatchArgs b) {
+    LV_BATCH_C(match_seq_body, kSuBSENSE, KF);
+}
 __global__ void __launch_bounds__(TPB) kb_match_cont(BatchArgs b) { LV_BATCH_C(match_cont_body, kSuBSENSE); }
 __global__ void __launch_bounds__(TPB) kb_fixpoint(BatchArgs b, uint32_t k) {
     for_batch_block(b, [&](const View& v, uint32_t bid, uint32_t nblk) {
@@ -2009,8 +2085,10 @@ static void frame_step_t(const View& v, const StepOptions& opt, hipStream_t st) 
     if (opt.fetch_depth == 4) hipLaunchKernelGGL((k_match<C, ALGO, 4>), dim3(g[kGridTile]), dim3(TPB), 0, st, v);
     else hipLaunchKernelGGL((k_match<C, ALGO, 2>), dim3(g[kGridTile]), dim3(TPB), 0, st, v);
     if (opt.timed_stage == 1) (void)hipEventRecord(opt.ev_end, st);
-    // the cooperative phase finishes what the sequential phase left (scans cut at seq_limit and
-    // pixels with more than kRepl overwritten slots); it exits at once when nothing was left
+    // phases B and C finish what phase A left (scans cut at seq_limit, pixels with more than kRepl
+    // overwritten slots); they exit at once when nothing was left
+    if (opt.fetch_depth == 4) hipLaunchKernelGGL((k_match_seq<C, ALGO, 4>), dim3(g[kGridCont]), dim3(TPB), 0, st, v);
+    else hipLaunchKernelGGL((k_match_seq<C, ALGO, 2>), dim3(g[kGridCont]), dim3(TPB), 0, st, v);
     hipLaunchKernelGGL((k_match_cont<C, ALGO>), dim3(g[kGridCont]), dim3(TPB), 0, st, v);
     for (uint32_t k = 1; k <= (uint32_t)opt.fixpoint_launches; ++k)
         hipLaunchKernelGGL((k_fixpoint<C, ALGO>), dim3(g[kGridFix]), dim3(TPB), 0, st, v, k);
@@ -2058,6 +2136,8 @@ void batch_step(const View* d_views, int ns, const uint32_t* const prefix[kNumGr
     if (opt.fetch_depth == 4) hipLaunchKernelGGL(kb_match<4>, dim3(total[kGridTile]), dim3(TPB), 0, st, args(kGridTile));
     else hipLaunchKernelGGL(kb_match<2>, dim3(total[kGridTile]), dim3(TPB), 0, st, args(kGridTile));
     if (opt.timed_stage == 1) (void)hipEventRecord(opt.ev_end, st);
+    if (opt.fetch_depth == 4) hipLaunchKernelGGL(kb_match_seq<4>, dim3(total[kGridCont]), dim3(TPB), 0, st, args(kGridCont));
+    else hipLaunchKernelGGL(kb_match_seq<2>, dim3(total[kGridCont]), dim3(TPB), 0, st, args(kGridCont));
     hipLaunchKernelGGL(kb_match_cont, dim3(total[kGridCont]), dim3(TPB), 0, st, args(kGridCont));
     for (uint32_t k = 1; k <= (uint32_t)opt.fixpoint_launches; ++k)
         hipLaunchKernelGGL(kb_fixpoint, dim3(total[kGridFix]), dim3(TPB), 0, st, args(kGridFix), k);

@@ -44,7 +44,8 @@ struct View {
     uint8_t* dec;            // W*H raw decisions (1 = foreground)
     uint32_t* wl0;           // worklists (n_roi entries each)
     uint32_t* wl1;
-    uint32_t* cont;          // pixels left to the cooperative phase of the match (n_roi entries)
+    uint32_t* cont;          // pixels left by the speculative match to the continuation (n_roi entries)
+    uint32_t* cont2;         // pixels left by the continuation to the cooperative phase
     unsigned long long* stamp;  // W*H worklist de-duplication stamps
     float *dsLT, *dsST;      // (W/8)*(H/8)*C
     Scalars* sc;
