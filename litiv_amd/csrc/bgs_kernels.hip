@@ -1504,82 +1504,133 @@ LV_D void holes_body(const View& v, uint32_t bid, uint32_t nblk) {
     }
 }
 
+// Post-processing masks are 0/255 bytes; the median and dilation kernels below work on four pixels
+// per 32-bit word. Words start at columns that are multiples of 4 (tile origins are), so when the
+// row pitch W is too a word is one aligned load or store; other widths assemble bytes.
+// in_image: 0xFF for each byte of the word at (gx..gx+3, gy) that lies inside the image
+LV_D uint32_t in_image_bytes(int W, int H, int gx, int gy) {
+    if (gy < 0 || gy >= H || gx + 3 < 0 || gx >= W) return 0u;
+    if (gx >= 0 && gx + 3 < W) return 0xFFFFFFFFu;
+    uint32_t m = 0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+        if (gx + i >= 0 && gx + i < W) m |= 0xFFu << (8 * i);
+    return m;
+}
+
+// bytes (gx..gx+3, gy) of a W x H byte image, `pad` for those outside it; gx % 4 == 0
+LV_D uint32_t load_mask_word(const uint8_t* m, int W, int H, int gx, int gy, uint32_t pad) {
+    const uint32_t fill = pad * 0x01010101u;
+    if (gy < 0 || gy >= H || gx + 3 < 0 || gx >= W) return fill;
+    const uint8_t* row = m + (size_t)gy * (size_t)W;
+    if ((W & 3) == 0) return *(const uint32_t*)(row + gx);  // gx >= 0 and gx + 3 < W here
+    uint32_t w = 0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int x = gx + i;
+        w |= (x >= 0 && x < W ? (uint32_t)row[x] : pad) << (8 * i);
+    }
+    return w;
+}
+
+LV_D void store_mask_word(uint8_t* m, int W, int H, int gx, int gy, uint32_t w) {
+    if (gy >= H || gx >= W) return;
+    uint8_t* row = m + (size_t)gy * (size_t)W;
+    if ((W & 3) == 0) {
+        *(uint32_t*)(row + gx) = w;
+        return;
+    }
+    for (int i = 0; i < 4 && gx + i < W; ++i) row[gx + i] = (uint8_t)(w >> (8 * i));
+}
+
+// byte window of two adjacent words: bytes s..s+3 of (lo, hi)
+LV_D uint32_t bytes_at(uint32_t hi, uint32_t lo, uint32_t s) { return __builtin_amdgcn_alignbyte(hi, lo, s); }
+
+// AND (OR) of the word whose bytes start 4 bytes into a, b, c and its shifts by up to 3 bytes either
+// way: the 7-wide erosion (dilation) of the 0/255 bytes of b along the row
+LV_D uint32_t erode7_word(uint32_t a, uint32_t b, uint32_t c) {
+    return b & bytes_at(b, a, 1) & bytes_at(b, a, 2) & bytes_at(b, a, 3) & bytes_at(c, b, 1) & bytes_at(c, b, 2) &
+           bytes_at(c, b, 3);
+}
+LV_D uint32_t dilate7_word(uint32_t a, uint32_t b, uint32_t c) {
+    return b | bytes_at(b, a, 1) | bytes_at(b, a, 2) | bytes_at(b, a, 3) | bytes_at(c, b, 1) | bytes_at(c, b, 2) |
+           bytes_at(c, b, 3);
+}
+
 // mask = raw | holes | erode7(PreFlood); last_fg = medianBlur(mask, k) (SUB:544-547; LOB:578 uses
 // the raw mask directly). Binary median with zero padding = majority count over the k x k window.
+// Counts are kept one per byte (at most 13 per column and 169 per window, so no byte carries into
+// the next) and the majority test adds 128 - need to every byte and keeps bit 7.
 template <bool SUBSENSE>
 LV_D void median_body(const View& v, uint32_t bid, uint32_t nblk) {
-    constexpr int HM = 6, HE = 3;                  // median halo (k <= 13), erosion halo
-    constexpr int MW = PW + 2 * HM, MH = PH + 2 * HM;  // mask region
-    constexpr int EW = MW + 2 * HE, EH = MH + 2 * HE;  // PreFlood region
-    __shared__ uint8_t s_pre[EH][EW];
-    __shared__ uint8_t s_eh[EH][MW];
-    __shared__ uint8_t s_mask[MH][MW];
-    __shared__ uint8_t s_col[PH][MW];
-    const int ntx = (v.W + PW - 1) / PW, nty = (v.H + PH - 1) / PH;
+    constexpr int HM = 6, HE = 3;           // median halo (k <= 13), erosion halo
+    constexpr int MWW = (PW + 16) / 4;      // mask words: columns x0-8 .. x0+PW+7
+    constexpr int MH = PH + 2 * HM;         // mask rows: y0-6 .. y0+PH+5
+    constexpr int EWW = (PW + 24) / 4;      // PreFlood words: columns x0-12 .. x0+PW+11
+    constexpr int EH = MH + 2 * HE;         // PreFlood rows: y0-9 .. y0+PH+8
+    constexpr int OWW = PW / 4;             // output words per row
+    __shared__ uint32_t s_pre[EH][EWW];
+    __shared__ uint32_t s_eh[EH][MWW];
+    __shared__ uint32_t s_mask[MH][MWW];
+    __shared__ uint32_t s_col[PH][MWW];
+    const int W = v.W, H = v.H;
+    const int ntx = (W + PW - 1) / PW;
     const int tid = (int)bid;
     const int x0 = (tid % ntx) * PW, y0 = (tid / ntx) * PH;
     const int k = v.median_k, r = k / 2, need = (k * k + 1) / 2;
-    // every window pass below slides its window along a run of outputs (one add, one remove)
     if constexpr (SUBSENSE) {
-        for (int i = threadIdx.x; i < EW * EH; i += TPB) {
-            const int ly = i / EW, lx = i % EW, gx = x0 + lx - HM - HE, gy = y0 + ly - HM - HE;
-            s_pre[ly][lx] = (gx >= 0 && gx < v.W && gy >= 0 && gy < v.H) ? v.pre[(size_t)gy * v.W + gx] : 255;
+        // erosion treats the outside of the image as 255
+        for (int i = threadIdx.x; i < EH * EWW; i += TPB) {
+            const int ly = i / EWW, wx = i % EWW;
+            s_pre[ly][wx] = load_mask_word(v.pre, W, H, x0 - 12 + 4 * wx, y0 - HM - HE + ly, 255u);
         }
         __syncthreads();
-        // horizontal 7-erosion of the 0/255 PreFlood: 0 when the window holds a 0
-        constexpr int HS = 4, HL = MW / HS;  // 4 runs of 19 per row
-        for (int it = threadIdx.x; it < EH * HS; it += TPB) {
-            const int ly = it / HS, xs = (it % HS) * HL;
-            int z = 0;
-            for (int b = 0; b < 2 * HE + 1; ++b) z += s_pre[ly][xs + b] == 0;
-            for (int lx = xs; lx < xs + HL; ++lx) {
-                s_eh[ly][lx] = z ? 0 : 255;
-                if (lx + 1 < xs + HL) z += (int)(s_pre[ly][lx + 2 * HE + 1] == 0) - (int)(s_pre[ly][lx] == 0);
-            }
+        // horizontal 7-erosion; mask word j (columns x0-8+4j..) is PreFlood word j+1
+        for (int i = threadIdx.x; i < EH * MWW; i += TPB) {
+            const int ly = i / MWW, j = i % MWW;
+            s_eh[ly][j] = erode7_word(s_pre[ly][j], s_pre[ly][j + 1], s_pre[ly][j + 2]);
         }
         __syncthreads();
     }
-    // mask = raw | holes | erode7(PreFlood) (LOBSTER: raw), as 0/1, with the vertical erosion
-    constexpr int VS = 2, VL = MH / VS;  // 2 runs of 14 per column
-    for (int it = threadIdx.x; it < MW * VS; it += TPB) {
-        const int lx = it % MW, ys = (it / MW) * VL;
-        int z = 0;
-        if constexpr (SUBSENSE)
-            for (int a = 0; a < 2 * HE + 1; ++a) z += s_eh[ys + a][lx] == 0;
-        for (int ly = ys; ly < ys + VL; ++ly) {
-            const int gx = x0 + lx - HM, gy = y0 + ly - HM;
-            uint8_t m = 0;
-            if (gx >= 0 && gx < v.W && gy >= 0 && gy < v.H) {
-                if constexpr (SUBSENSE) m = v.mh[(size_t)gy * v.W + gx] | (z ? 0 : 255);
-                else m = v.raw[(size_t)gy * v.W + gx];
-            }
-            s_mask[ly][lx] = m ? 1 : 0;
-            if constexpr (SUBSENSE)
-                if (ly + 1 < ys + VL) z += (int)(s_eh[ly + 2 * HE + 1][lx] == 0) - (int)(s_eh[ly][lx] == 0);
+    // the mask as 0/1 bytes, zero outside the image; mask row ly is PreFlood row ly+3
+    for (int i = threadIdx.x; i < MH * MWW; i += TPB) {
+        const int ly = i / MWW, j = i % MWW, gx = x0 - 8 + 4 * j, gy = y0 - HM + ly;
+        uint32_t m;
+        if constexpr (SUBSENSE) {
+            uint32_t e = s_eh[ly][j];
+#pragma unroll
+            for (int a = 1; a <= 2 * HE; ++a) e &= s_eh[ly + a][j];
+            m = (load_mask_word(v.mh, W, H, gx, gy, 0u) | e) & in_image_bytes(W, H, gx, gy);
+        } else {
+            m = load_mask_word(v.raw, W, H, gx, gy, 0u);
         }
+        s_mask[ly][j] = m & 0x01010101u;
     }
     __syncthreads();
-    // column counts over k rows, one run of PH outputs per column
-    for (int lx = threadIdx.x; lx < MW; lx += TPB) {
-        int c = 0;
-        for (int a = -r; a <= r; ++a) c += s_mask[HM + a][lx];
-        for (int ly = 0; ly < PH; ++ly) {
-            s_col[ly][lx] = (uint8_t)c;
-            if (ly + 1 < PH) c += (int)s_mask[HM + ly + 1 + r][lx] - (int)s_mask[HM + ly - r][lx];
-        }
+    // column counts over k rows
+    for (int i = threadIdx.x; i < PH * MWW; i += TPB) {
+        const int ly = i / MWW, j = i % MWW;
+        uint32_t c = 0;
+        for (int a = HM + ly - r; a <= HM + ly + r; ++a) c += s_mask[a][j];
+        s_col[ly][j] = c;
     }
     __syncthreads();
-    // row counts over k columns and the majority decision, runs of 16 outputs
-    constexpr int RS = PW / 16;
-    for (int it = threadIdx.x; it < PH * RS; it += TPB) {
-        const int ly = it / RS, xs = (it % RS) * 16, gy = y0 + ly;
-        int c = 0;
-        for (int b = -r; b <= r; ++b) c += s_col[ly][xs + HM + b];
-        for (int lx = xs; lx < xs + 16; ++lx) {
-            const int gx = x0 + lx;
-            if (gx < v.W && gy < v.H) v.last_fg[(size_t)gy * v.W + gx] = c >= need ? 255 : 0;
-            if (lx + 1 < xs + 16) c += (int)s_col[ly][lx + 1 + HM + r] - (int)s_col[ly][lx + HM - r];
+    // row counts over k columns: output word q (columns x0+4q..) starts at byte 4q+8 of the counts
+    const uint32_t bias = (uint32_t)(128 - need) * 0x01010101u;
+    for (int i = threadIdx.x; i < PH * OWW; i += TPB) {
+        const int ly = i / OWW, q = i % OWW;
+        uint32_t w[5];
+#pragma unroll
+        for (int u = 0; u < 5; ++u) w[u] = s_col[ly][q + u];
+        uint32_t s = 0;
+#pragma unroll
+        for (int o = -HM; o <= HM; ++o) {
+            const int b = 8 + o, wi = b >> 2, sh = b & 3;
+            const uint32_t t = sh ? bytes_at(w[wi + 1], w[wi], (uint32_t)sh) : w[wi];
+            s += (o >= -r && o <= r) ? t : 0u;
         }
+        const uint32_t bits = ((s + bias) >> 7) & 0x01010101u;
+        store_mask_word(v.last_fg, W, H, x0 + 4 * q, y0 + ly, bits * 255u);
     }
 }
 
@@ -1587,22 +1638,24 @@ LV_D void median_body(const View& v, uint32_t bid, uint32_t nblk) {
 // output mask. addWeighted(.., CV_32F) is evaluated as the double-precision scalar form.
 LV_D void dilate_final_body(const View& v, uint32_t bid, uint32_t nblk) {
     uint8_t* out_mask = v.out_mask;
-    constexpr int H3 = 3, LW = PW + 2 * H3, LH = PH + 2 * H3;
-    __shared__ uint8_t s_fg[LH][LW];
-    __shared__ uint8_t s_h[LH][PW];
-    const int ntx = (v.W + PW - 1) / PW, nty = (v.H + PH - 1) / PH;
+    constexpr int H3 = 3;
+    constexpr int FWW = (PW + 8) / 4;  // last_fg words: columns x0-4 .. x0+PW+3
+    constexpr int LH = PH + 2 * H3;    // rows y0-3 .. y0+PH+2
+    constexpr int OWW = PW / 4;
+    __shared__ uint32_t s_fg[LH][FWW];
+    __shared__ uint32_t s_h[LH][OWW];
+    const int W = v.W, H = v.H;
+    const int ntx = (W + PW - 1) / PW;
     const int tid = (int)bid;
     const int x0 = (tid % ntx) * PW, y0 = (tid / ntx) * PH;
-    for (int i = threadIdx.x; i < LW * LH; i += TPB) {
-        const int ly = i / LW, lx = i % LW, gx = x0 + lx - H3, gy = y0 + ly - H3;
-        s_fg[ly][lx] = (gx >= 0 && gx < v.W && gy >= 0 && gy < v.H) ? v.last_fg[(size_t)gy * v.W + gx] : 0;
+    for (int i = threadIdx.x; i < LH * FWW; i += TPB) {
+        const int ly = i / FWW, wx = i % FWW;
+        s_fg[ly][wx] = load_mask_word(v.last_fg, W, H, x0 - 4 + 4 * wx, y0 - H3 + ly, 0u);
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < LH * PW; i += TPB) {
-        const int ly = i / PW, lx = i % PW;
-        uint8_t m = 0;
-        for (int b = 0; b < 2 * H3 + 1; ++b) m = s_fg[ly][lx + b] > m ? s_fg[ly][lx + b] : m;
-        s_h[ly][lx] = m;
+    for (int i = threadIdx.x; i < LH * OWW; i += TPB) {
+        const int ly = i / OWW, q = i % OWW;
+        s_h[ly][q] = dilate7_word(s_fg[ly][q], s_fg[ly][q + 1], s_fg[ly][q + 2]);
     }
     __syncthreads();
     const uint32_t frame_no = cur_frame(v), mvn = v.mv_avgs;
@@ -1610,20 +1663,51 @@ LV_D void dilate_final_body(const View& v, uint32_t bid, uint32_t nblk) {
     const float aST = 1.0f / (float)(frame_no < mvn / 4u ? frame_no : mvn / 4u);
     const double alt = (double)(1.0f - aLT), blt = (1.0 / 255) * (double)aLT;
     const double ast = (double)(1.0f - aST), bst = (1.0 / 255) * (double)aST;
-    for (int i = threadIdx.x; i < PW * PH; i += TPB) {
-        const int ly = i / PW, lx = i % PW, gx = x0 + lx, gy = y0 + ly;
-        if (gx >= v.W || gy >= v.H) continue;
-        uint8_t d = 0;
-        for (int a = 0; a < 2 * H3 + 1; ++a) d = s_h[ly + a][lx] > d ? s_h[ly + a][lx] : d;
-        const size_t p = (size_t)gy * v.W + gx;
-        const uint8_t inv = (uint8_t)~d;
-        v.blinks[p] = (uint8_t)(v.blinks[p] & v.last_dil_inv[p] & inv);
-        v.last_dil_inv[p] = inv;
-        const uint8_t fg = s_fg[ly + H3][lx + H3];
-        const double f = (double)fg;
-        v.FinLT[p] = (float)((double)v.FinLT[p] * alt + f * blt + 0.0);
-        v.FinST[p] = (float)((double)v.FinST[p] * ast + f * bst + 0.0);
-        if (out_mask) out_mask[p] = fg;
+    for (int i = threadIdx.x; i < PH * OWW; i += TPB) {
+        const int ly = i / OWW, q = i % OWW, gx = x0 + 4 * q, gy = y0 + ly;
+        if (gx >= W || gy >= H) continue;
+        uint32_t d = s_h[ly][q];
+#pragma unroll
+        for (int a = 1; a <= 2 * H3; ++a) d |= s_h[ly + a][q];
+        const uint32_t inv = ~d, fg = s_fg[ly + H3][q + 1];
+        const size_t p = (size_t)gy * W + gx;
+        if ((W & 3) == 0) {
+            uint32_t* bl = (uint32_t*)(v.blinks + p);
+            uint32_t* ldi = (uint32_t*)(v.last_dil_inv + p);
+            *bl = *bl & *ldi & inv;
+            *ldi = inv;
+            float4* lt = (float4*)(v.FinLT + p);
+            float4* st = (float4*)(v.FinST + p);
+            float4 a = *lt, b = *st;
+            const double f0 = (double)(fg & 255u), f1 = (double)((fg >> 8) & 255u);
+            const double f2 = (double)((fg >> 16) & 255u), f3 = (double)(fg >> 24);
+            a.x = (float)((double)a.x * alt + f0 * blt + 0.0);
+            a.y = (float)((double)a.y * alt + f1 * blt + 0.0);
+            a.z = (float)((double)a.z * alt + f2 * blt + 0.0);
+            a.w = (float)((double)a.w * alt + f3 * blt + 0.0);
+            b.x = (float)((double)b.x * ast + f0 * bst + 0.0);
+            b.y = (float)((double)b.y * ast + f1 * bst + 0.0);
+            b.z = (float)((double)b.z * ast + f2 * bst + 0.0);
+            b.w = (float)((double)b.w * ast + f3 * bst + 0.0);
+            *lt = a;
+            *st = b;
+            if (out_mask) {
+                if (((uintptr_t)out_mask & 3u) == 0) *(uint32_t*)(out_mask + p) = fg;  // caller's buffer
+                else
+                    for (int u = 0; u < 4; ++u) out_mask[p + u] = (uint8_t)(fg >> (8 * u));
+            }
+        } else {
+            for (int u = 0; u < 4 && gx + u < W; ++u) {
+                const size_t pu = p + u;
+                const uint8_t iu = (uint8_t)(inv >> (8 * u)), fu = (uint8_t)(fg >> (8 * u));
+                v.blinks[pu] = (uint8_t)(v.blinks[pu] & v.last_dil_inv[pu] & iu);
+                v.last_dil_inv[pu] = iu;
+                const double f = (double)fu;
+                v.FinLT[pu] = (float)((double)v.FinLT[pu] * alt + f * blt + 0.0);
+                v.FinST[pu] = (float)((double)v.FinST[pu] * ast + f * bst + 0.0);
+                if (out_mask) out_mask[pu] = fu;
+            }
+        }
     }
 }
 
