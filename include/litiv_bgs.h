@@ -158,6 +158,12 @@ int lvbgs_set_match_schedule(lvbgs_ctx* ctx, int seq_limit, int cont_blocks);
  * written to host_out[4*p .. 4*p+3] (phase 1 = apply, 2 = refresh). */
 int lvbgs_debug_rng(lvbgs_ctx* ctx, uint32_t frame, uint32_t phase, uint32_t event, uint32_t n, uint32_t* host_out);
 
+/* Post-processing self-check (SuBSENSE): runs the hole filling and median stage of
+ * BackgroundSubtractorSuBSENSE.cpp:540-547 on a caller-given W x H raw mask (0/255, tightly packed host
+ * buffer) and writes the resulting median-filtered mask to host_last_fg. Replaces the context's
+ * last foreground mask; for tests only. */
+int lvbgs_debug_postprocess(lvbgs_ctx* ctx, const uint8_t* host_raw, uint8_t* host_last_fg);
+
 /* ---- batches (no counterpart in the reference: its changedet app runs one sequence per host
  * thread, apps/changedet/src/main.cpp:148-152) ----
  * A batch steps several initialized SuBSENSE contexts (any mix of sizes, 1 or 3 channels) together:

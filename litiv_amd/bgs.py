@@ -123,6 +123,7 @@ def lib():
         L.lvbgs_set_fixpoint_schedule.argtypes = [vp, i, i]
         L.lvbgs_set_match_schedule.argtypes = [vp, i, i]
         L.lvbgs_debug_rng.argtypes = [vp, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, vp]
+        L.lvbgs_debug_postprocess.argtypes = [vp, vp, vp]
         L.lvbgs_initialize_device.argtypes = [vp, vp, i, i, i, vp, sz]
         L.lvbgs_model_bytes.argtypes = [i, C.POINTER(Params), i, i, i]
         L.lvbgs_model_bytes.restype = sz
@@ -357,6 +358,14 @@ class BackgroundSubtractor:
     def debug_rng(self, frame, phase, event, n):
         out = np.empty((n, 4), np.uint32)
         self._check(lib().lvbgs_debug_rng(self._h, frame, phase, event, n, out.ctypes.data))
+        return out
+
+    def debug_postprocess(self, raw):
+        """Hole filling + median of a raw 0/255 mask (SuBSENSE .cpp:540-547) on the device; returns
+        the filtered mask and leaves it as the context's last foreground mask."""
+        raw = np.ascontiguousarray(raw, dtype=np.uint8)
+        out = np.empty_like(raw)
+        self._check(lib().lvbgs_debug_postprocess(self._h, raw.ctypes.data, out.ctypes.data))
         return out
 
 

@@ -150,9 +150,14 @@ static void carve(lvbgs_ctx* c, Arena& ar) {
     float** maps[] = {&v.T, &v.R, &v.V, &v.MLD, &v.DminLT, &v.DminST, &v.RawLT, &v.RawST, &v.FinLT, &v.FinST};
     for (auto m : maps) *m = sub ? ar.take<float>(npx) : nullptr;
     uint8_t** u8s[] = {&v.unstable, &v.blinks, &v.last_raw, &v.last_raw_blink, &v.last_fg, &v.last_dil_inv,
-                       &v.raw, &v.pre, &v.mh, &v.dec};
+                       &v.raw, &v.dec};
     for (auto m : u8s) *m = ar.take<uint8_t>(npx);
-    v.label = ar.take<uint32_t>(npx);
+    const size_t nz = (size_t)((c->W + 63) / 64) * (size_t)((c->H + 63) / 64);
+    v.zbits = ar.take<uint64_t>(nz * 64);
+    v.hbits = ar.take<uint64_t>(nz * 64);
+    v.zedge = ar.take<uint8_t>(nz * 256);
+    v.zpar = ar.take<uint32_t>(nz * 128);
+    v.zcount = ar.take<uint32_t>(nz);
     v.desc_cur = ar.take<uint16_t>(npx * C);
     v.rec = ar.take<uint32_t>(npx);
     v.mres = ar.take<uint32_t>(npx);
@@ -731,6 +736,19 @@ extern "C" int lvbgs_debug_rng(lvbgs_ctx* c, uint32_t frame, uint32_t phase, uin
     int rc = hip_check(c, hipMemcpyAsync(host_out, d, (size_t)n * 16, hipMemcpyDeviceToHost, c->stream), "rng copy");
     if (!rc) rc = hip_check(c, hipStreamSynchronize(c->stream), "sync");
     hipFree(d);
+    return rc;
+}
+
+extern "C" int lvbgs_debug_postprocess(lvbgs_ctx* c, const uint8_t* host_raw, uint8_t* host_last_fg) {
+    if (!c || !host_raw || !host_last_fg) return LVBGS_ERR_ARG;
+    if (!c->initialized) return set_err(c, LVBGS_ERR_STATE, "algo & model must be initialized first");
+    if (c->algo != LVBGS_SUBSENSE) return set_err(c, LVBGS_ERR_ARG, "post-processing check is for SuBSENSE");
+    hipSetDevice(c->device);
+    int rc = hip_check(c, hipMemcpyAsync(c->v.raw, host_raw, c->npx, hipMemcpyHostToDevice, c->stream), "raw copy");
+    if (rc) return rc;
+    launch_postprocess_probe(c->v, c->stream);
+    rc = hip_check(c, hipMemcpyAsync(host_last_fg, c->v.last_fg, c->npx, hipMemcpyDeviceToHost, c->stream), "mask copy");
+    if (!rc) rc = hip_check(c, hipStreamSynchronize(c->stream), "sync");
     return rc;
 }
 

@@ -1330,101 +1330,192 @@ LV_D void copy_last_color_body(const View& v, uint32_t bid, uint32_t nblk) {
 // ----------------------------------------------------------------------------------------------
 constexpr int PW = 64, PH = 16;  // post-processing tile; 256 threads x 4 pixels
 
-// --- hole fill: union-find labelling of the 0-pixels of PreFlood (4-connectivity) ---
-constexpr int CT = 32;  // CCL tile (32x32)
+// --- hole fill (SUB:540-543) on 64x64 zones, one wavefront per zone --------------------------
+// PreFlood = morphologyEx(raw, CLOSE, 3x3) = erode3(dilate3(raw)) with neutral borders (SUB:540);
+// holes = its 0-pixels not 4-connected through 0-pixels to pixel (0,0) (floodFill + bitwise_not,
+// SUB:541-543). Lane y of a zone's wavefront holds row y of the zone as a 64-bit mask (bit x =
+// column x). A zone's 0-pixels split into components that touch the zone border, numbered per
+// zone, and components inside it, which are holes. The border components of all zones are joined
+// across zone edges by union-find; a border component is outside iff its root is the root of the
+// component holding pixel (0,0).
+constexpr int ZS = 64;                  // zone side
+constexpr int ZPB = TPB / 64;           // zones per block, one per wavefront
+constexpr uint32_t kZoneComps = 128;    // border components per zone (at most 126 of 252 pixels)
 
-// One 32x32 tile: PreFlood = morphologyEx(raw, CLOSE, 3x3) = erode3(dilate3(raw)) with neutral
-// borders (SUB:540), written out and labelled here: union-find of the tile's 0-pixels.
-LV_D void ccl_local_body(const View& v, uint32_t bid, uint32_t nblk) {
-    constexpr int RW = CT + 4, DW = CT + 2;
-    __shared__ uint32_t lab[CT * CT];
-    __shared__ uint8_t s_raw[RW][RW];
-    __shared__ uint8_t s_dil[DW][DW];
-    const int ntx = (v.W + CT - 1) / CT;
-    const int x0 = ((int)bid % ntx) * CT, y0 = ((int)bid / ntx) * CT;
-    for (int i = threadIdx.x; i < RW * RW; i += TPB) {
-        const int ly = i / RW, lx = i % RW, gx = x0 + lx - 2, gy = y0 + ly - 2;
-        s_raw[ly][lx] = (gx >= 0 && gx < v.W && gy >= 0 && gy < v.H) ? v.raw[(size_t)gy * v.W + gx] : 0;
+LV_D uint64_t brev64(uint64_t x) {
+    return ((uint64_t)__builtin_bitreverse32((uint32_t)x) << 32) | __builtin_bitreverse32((uint32_t)(x >> 32));
+}
+LV_D uint64_t lane_bcast(uint64_t x, int l) {
+    return ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(x >> 32), l) << 32) |
+           (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)x, l);
+}
+LV_D uint64_t from_lane_above(uint64_t x, int d, int lane) {  // lane - d, 0 for the first d lanes
+    const uint64_t r = __shfl_up(x, (unsigned)d, 64);
+    return lane >= d ? r : 0ull;
+}
+LV_D uint64_t from_lane_below(uint64_t x, int d, int lane) {  // lane + d, 0 for the last d lanes
+    const uint64_t r = __shfl_down(x, (unsigned)d, 64);
+    return lane + d < 64 ? r : 0ull;
+}
+// wavefront-local LDS exchange: the lanes' stores above are visible to the loads below
+LV_D void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// the runs of z along the row that hold a bit of s (s within z). Adding s to z carries through
+// the run above each seed bit and z & ~(z + s) keeps the bits the carry cleared; the same on the
+// bit-reversed row covers the run below.
+LV_D uint64_t fill_row(uint64_t z, uint64_t zr, uint64_t s) {
+    const uint64_t up = (z & ~(z + s)) | s;
+    const uint64_t ur = brev64(up);
+    return brev64((zr & ~(zr + ur)) | ur);
+}
+// the runs of z down and up the columns that hold a bit of s: segmented OR scans over the lanes,
+// g = reached at this row from the rows above within the window, p = window all in z
+LV_D uint64_t fill_cols(uint64_t z, uint64_t s, int lane) {
+    uint64_t g = s, p = z;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint64_t gu = from_lane_above(g, d, lane), pu = from_lane_above(p, d, lane);
+        g |= p & gu;
+        p &= pu;
     }
-    __syncthreads();
-    for (int i = threadIdx.x; i < DW * DW; i += TPB) {
-        const int ly = i / DW, lx = i % DW, gx = x0 + lx - 1, gy = y0 + ly - 1;
-        uint8_t m = 255;  // outside the image: neutral for the erosion
-        if (gx >= 0 && gx < v.W && gy >= 0 && gy < v.H) {
-            m = 0;
-            for (int a = 0; a < 3; ++a)
-                for (int b = 0; b < 3; ++b) m = s_raw[ly + a][lx + b] > m ? s_raw[ly + a][lx + b] : m;
+    p = z;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint64_t gd = from_lane_below(g, d, lane), pd = from_lane_below(p, d, lane);
+        g |= p & gd;
+        p &= pd;
+    }
+    return g;
+}
+// the 4-connected components of z holding a bit of s; each round adds at least one bit or ends
+LV_D uint64_t flood(uint64_t z, uint64_t zr, uint64_t s, int lane) {
+    uint64_t f = s & z;
+    while (true) {
+        const uint64_t g = fill_cols(z, fill_row(z, zr, f), lane);
+        if (__ballot(g != f) == 0ull) return g;
+        f = g;
+    }
+}
+
+// raw bits of columns x0-2, x0-1, x0+64, x0+65 of row gy (bits 0..3)
+LV_D uint32_t raw_halo(const uint8_t* raw, int W, int H, int x0, int gy) {
+    if (gy < 0 || gy >= H) return 0u;
+    const uint8_t* row = raw + (size_t)gy * (size_t)W;
+    const int xs[4] = {x0 - 2, x0 - 1, x0 + ZS, x0 + ZS + 1};
+    uint32_t h = 0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+        if (xs[i] >= 0 && xs[i] < W && row[xs[i]] != 0) h |= 1u << i;
+    return h;
+}
+// 3-wide dilation along a row: core columns x0..x0+63 and the halo bits above -> dilated core and
+// dilated halo (bit 0: column x0-1, bit 1: column x0+64)
+LV_D void dilate3_row(uint64_t c, uint32_t h, uint64_t& d, uint32_t& dh) {
+    d = c | (c << 1) | (uint64_t)((h >> 1) & 1u) | (c >> 1) | ((uint64_t)((h >> 2) & 1u) << 63);
+    const uint32_t left = (h & 1u) | ((h >> 1) & 1u) | (uint32_t)(c & 1ull);
+    const uint32_t right = (uint32_t)(c >> 63) | ((h >> 2) & 1u) | ((h >> 3) & 1u);
+    dh = left | (right << 1);
+}
+
+// closing of the zone, its 0-pixels, and their border components
+LV_D void zone_close_body(const View& v, uint32_t bid, uint32_t nblk) {
+    __shared__ uint64_t s_d[ZPB][ZS + 4];   // row-dilated raw rows y0-2 .. y0+65
+    __shared__ uint32_t s_dx[ZPB][ZS + 4];
+    const int W = v.W, H = v.H;
+    const int lane = (int)(threadIdx.x & 63u), wv = (int)(threadIdx.x >> 6);
+    const int nzx = (W + ZS - 1) / ZS, nz = nzx * ((H + ZS - 1) / ZS);
+    const int z = (int)bid * ZPB + wv;
+    if (z >= nz) return;  // whole wavefronts only; nothing below synchronizes the block
+    const int x0 = (z % nzx) * ZS, y0 = (z / nzx) * ZS;
+    const uint8_t* raw = v.raw;
+    // raw rows y0-2+r as 64-bit masks: lane = column for the loads, one ballot per row; lane y
+    // keeps row y, lanes 0..3 also rows 64..67
+    const int gx = x0 + lane;
+    uint64_t c0 = 0, c1 = 0;
+#pragma unroll 17
+    for (int r = 0; r < ZS + 4; ++r) {
+        const int gy = y0 - 2 + r;
+        const uint8_t b = (gx < W && gy >= 0 && gy < H) ? raw[(size_t)gy * (size_t)W + gx] : (uint8_t)0;
+        const uint64_t m = __ballot(b != 0);
+        if (r < ZS) c0 = lane == r ? m : c0;
+        else c1 = lane == r - ZS ? m : c1;
+    }
+    const uint32_t h0 = raw_halo(raw, W, H, x0, y0 - 2 + lane);
+    const uint32_t h1 = lane < 4 ? raw_halo(raw, W, H, x0, y0 - 2 + ZS + lane) : 0u;
+    {
+        uint64_t d;
+        uint32_t dh;
+        dilate3_row(c0, h0, d, dh);
+        s_d[wv][lane] = d;
+        s_dx[wv][lane] = dh;
+        dilate3_row(c1, h1, d, dh);
+        if (lane < 4) {
+            s_d[wv][ZS + lane] = d;
+            s_dx[wv][ZS + lane] = dh;
         }
-        s_dil[ly][lx] = m;
     }
-    __syncthreads();
-    int any_fg = 0;
-    for (int i = threadIdx.x; i < CT * CT; i += TPB) {
-        const int ly = i / CT, lx = i % CT, gx = x0 + lx, gy = y0 + ly;
-        bool bgpx = false;
-        if (gx < v.W && gy < v.H) {
-            uint8_t m = 255;
-            for (int a = 0; a < 3; ++a)
-                for (int b = 0; b < 3; ++b) m = s_dil[ly + a][lx + b] < m ? s_dil[ly + a][lx + b] : m;
-            v.pre[(size_t)gy * v.W + gx] = m;
-            bgpx = m == 0;
+    wave_sync();
+    // lane y: zone row y = dilated-row index y+2; erosion over dilated rows y+1..y+3, each the OR
+    // of three row-dilated rows; outside the image the dilation is 255 (neutral for the erosion)
+    const uint64_t in_cols = W - x0 >= ZS ? ~0ull : ((1ull << (W - x0)) - 1ull);
+    const uint32_t out_halo = (x0 == 0 ? 1u : 0u) | (x0 + ZS >= W ? 2u : 0u);
+    uint64_t e = ~0ull;
+    uint32_t ex = 3u;
+#pragma unroll
+    for (int a = 1; a <= 3; ++a) {
+        const int r = lane + a, gy = y0 - 2 + r;
+        uint64_t d = s_d[wv][r - 1] | s_d[wv][r] | s_d[wv][r + 1];
+        uint32_t dx = s_dx[wv][r - 1] | s_dx[wv][r] | s_dx[wv][r + 1];
+        if (gy < 0 || gy >= H) {
+            d = ~0ull;
+            dx = 3u;
+        } else {
+            d |= ~in_cols;
+            dx |= out_halo;
         }
-        lab[i] = bgpx ? (uint32_t)i : 0xFFFFFFFFu;
-        any_fg |= bgpx ? 0 : 1;
+        e &= d;
+        ex &= dx;
     }
-    // a tile inside the image with no foreground is one 4-connected component rooted at its
-    // first pixel (the minimum index, as the union below would find)
-    if (!__syncthreads_or(any_fg)) {
-        const uint32_t root = (uint32_t)(y0 * v.W + x0);
-        for (int i = threadIdx.x; i < CT * CT; i += TPB)
-            v.label[(size_t)(y0 + i / CT) * v.W + (x0 + i % CT)] = root;
-        return;
+    const uint64_t pre = e & ((e << 1) | (uint64_t)(ex & 1u)) & ((e >> 1) | ((uint64_t)(ex >> 1) << 63));
+    const uint64_t zm = y0 + lane < H ? (~pre & in_cols) : 0ull;
+    v.zbits[(size_t)z * ZS + lane] = zm;
+    // border components in the order top row, left column, right column, bottom row
+    const uint64_t zr = brev64(zm);
+    uint64_t remT = lane_bcast(zm, 0), remB = lane_bcast(zm, 63);
+    uint64_t remL = __ballot((zm & 1ull) != 0), remR = __ballot((zm >> 63) != 0);
+    uint32_t cT = 255u, cB = 255u, cL = 255u, cR = 255u, k = 0;
+    while ((remT | remB | remL | remR) != 0ull) {
+        uint64_t s;
+        if (remT) s = lane == 0 ? 1ull << __builtin_ctzll(remT) : 0ull;
+        else if (remL) s = lane == __builtin_ctzll(remL) ? 1ull : 0ull;
+        else if (remR) s = lane == __builtin_ctzll(remR) ? 1ull << 63 : 0ull;
+        else s = lane == 63 ? 1ull << __builtin_ctzll(remB) : 0ull;
+        const uint64_t f = flood(zm, zr, s, lane);
+        const uint64_t fT = lane_bcast(f, 0), fB = lane_bcast(f, 63);
+        const uint64_t fL = __ballot((f & 1ull) != 0), fR = __ballot((f >> 63) != 0);
+        if ((fT >> lane) & 1ull) cT = k;
+        if ((fB >> lane) & 1ull) cB = k;
+        if ((fL >> lane) & 1ull) cL = k;
+        if ((fR >> lane) & 1ull) cR = k;
+        remT &= ~fT;
+        remB &= ~fB;
+        remL &= ~fL;
+        remR &= ~fR;
+        ++k;
     }
-    for (int i = threadIdx.x; i < CT * CT; i += TPB) {
-        if (lab[i] == 0xFFFFFFFFu) continue;
-        const int lx = i % CT, ly = i / CT;
-        const int nb[2] = {lx + 1 < CT ? i + 1 : -1, ly + 1 < CT ? i + CT : -1};
-        for (int k = 0; k < 2; ++k) {
-            if (nb[k] < 0) continue;
-            uint32_t a = (uint32_t)i, b = (uint32_t)nb[k];
-            if (__hip_atomic_load(&lab[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) == 0xFFFFFFFFu) continue;
-            bool done = false;
-            while (!done) {
-                while (true) {
-                    const uint32_t pa = __hip_atomic_load(&lab[a], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-                    if (pa == a) break;
-                    a = pa;
-                }
-                while (true) {
-                    const uint32_t pb = __hip_atomic_load(&lab[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-                    if (pb == b) break;
-                    b = pb;
-                }
-                if (a < b) {
-                    const uint32_t old = atomicMin(&lab[b], a);
-                    done = old == b;
-                    b = old;
-                } else if (b < a) {
-                    const uint32_t old = atomicMin(&lab[a], b);
-                    done = old == a;
-                    a = old;
-                } else {
-                    done = true;
-                }
-            }
-        }
-    }
-    __syncthreads();
-    for (int i = threadIdx.x; i < CT * CT; i += TPB) {
-        const int gx = x0 + i % CT, gy = y0 + i / CT;
-        if (gx >= v.W || gy >= v.H) continue;
-        uint32_t a = lab[i];
-        if (a != 0xFFFFFFFFu) {
-            while (lab[a] != a) a = lab[a];
-            a = (uint32_t)((y0 + (int)a / CT) * v.W + (x0 + (int)a % CT));
-        }
-        v.label[(size_t)gy * v.W + gx] = a;
-    }
+    uint8_t* edge = v.zedge + (size_t)z * 256;
+    edge[lane] = (uint8_t)cT;
+    edge[64 + lane] = (uint8_t)cB;
+    edge[128 + lane] = (uint8_t)cL;
+    edge[192 + lane] = (uint8_t)cR;
+    const uint32_t base = (uint32_t)z * kZoneComps;
+    if ((uint32_t)lane < k) v.zpar[base + lane] = base + lane;
+    if ((uint32_t)lane + 64u < k) v.zpar[base + lane + 64] = base + lane + 64;
+    if (lane == 0) v.zcount[z] = k;
 }
 
 LV_D uint32_t g_find(uint32_t* lab, uint32_t a) {
@@ -1432,6 +1523,17 @@ LV_D uint32_t g_find(uint32_t* lab, uint32_t a) {
         const uint32_t pa = __hip_atomic_load(&lab[a], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (pa == a) return a;
         a = pa;
+    }
+}
+
+// the same without concurrent unions: path halving (every store is an ancestor of the node)
+LV_D uint32_t g_find_halving(uint32_t* lab, uint32_t a) {
+    while (true) {
+        const uint32_t pa = __hip_atomic_load(&lab[a], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (pa == a) return a;
+        const uint32_t ga = __hip_atomic_load(&lab[pa], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (ga != pa) __hip_atomic_store(&lab[a], ga, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        a = ga;
     }
 }
 
@@ -1454,54 +1556,55 @@ LV_D void g_union(uint32_t* lab, uint32_t a, uint32_t b) {
     }
 }
 
-// unions across CCL tile borders (right column and bottom row of every tile)
-LV_D void ccl_border_body(const View& v, uint32_t bid, uint32_t nblk) {
-    const int ntx = (v.W + CT - 1) / CT, nty = (v.H + CT - 1) / CT;
-    const uint32_t per_tile = 2 * CT;
-    const uint32_t i = bid * TPB + threadIdx.x;
-    if (i >= (uint32_t)(ntx * nty) * per_tile) return;
-    const int t = (int)(i / per_tile), k = (int)(i % per_tile);
-    const int x0 = (t % ntx) * CT, y0 = (t / ntx) * CT;
-    int ax, ay, bx, by;
-    if (k < CT) { ax = x0 + CT - 1; ay = y0 + k; bx = ax + 1; by = ay; }
-    else { ax = x0 + (k - CT); ay = y0 + CT - 1; bx = ax; by = ay + 1; }
-    if (ax >= v.W || ay >= v.H || bx >= v.W || by >= v.H) return;
-    const uint32_t a = (uint32_t)(ay * v.W + ax), b = (uint32_t)(by * v.W + bx);
-    if (v.pre[a] != 0 || v.pre[b] != 0) return;
-    // labels still hold tile-local roots; the previous pixel along the edge usually carries the
-    // same pair of local roots, in which case its union already covers this one
-    const uint32_t la = v.label[a], lb = v.label[b];
-    if (k % CT != 0) {
-        const uint32_t pa = k < CT ? a - (uint32_t)v.W : a - 1u, pb = k < CT ? b - (uint32_t)v.W : b - 1u;
-        if (v.pre[pa] == 0 && v.pre[pb] == 0 && v.label[pa] == la && v.label[pb] == lb) return;
-    }
-    g_union(v.label, la, lb);
+// joins the components facing each other across one zone edge (lane = position along the edge);
+// a lane whose pair repeats the previous lane's leaves the union to that lane
+LV_D void zone_link(uint32_t* par, uint32_t za, uint32_t zb, uint32_t a, uint32_t b, int lane) {
+    const uint32_t pa = (uint32_t)__shfl_up((int)a, 1u, 64), pb = (uint32_t)__shfl_up((int)b, 1u, 64);
+    const bool dup = lane > 0 && pa == a && pb == b;
+    if (a != 255u && b != 255u && !dup) g_union(par, za * kZoneComps + a, zb * kZoneComps + b);
 }
 
-// holes = 0-pixels of PreFlood not connected to pixel (0,0) (floodFill + bitwise_not, SUB:541-543);
-// mh = raw | holes
-// One 32x32 tile per block. A background tile's pixels all carry the tile origin as their label,
-// so the origin's root is found once and shared.
-LV_D void holes_body(const View& v, uint32_t bid, uint32_t nblk) {
-    __shared__ uint32_t s_root;
-    const int ntx = (v.W + CT - 1) / CT;
-    const int x0 = ((int)bid % ntx) * CT, y0 = ((int)bid / ntx) * CT;
-    const uint32_t origin = (uint32_t)(y0 * v.W + x0);
-    if (threadIdx.x == 0) s_root = v.pre[origin] == 0 ? g_find(v.label, origin) : 0xFFFFFFFFu;
-    __syncthreads();
-    const uint32_t root0 = s_root;
-    for (int i = threadIdx.x; i < CT * CT; i += TPB) {
-        const int gx = x0 + i % CT, gy = y0 + i / CT;
-        if (gx >= v.W || gy >= v.H) continue;
-        const uint32_t p = (uint32_t)(gy * v.W + gx);
-        uint8_t hole = 0;
-        if (v.pre[p] == 0) {
-            const uint32_t l = v.label[p];
-            const uint32_t r = (l == origin && root0 != 0xFFFFFFFFu) ? root0 : g_find(v.label, l);
-            hole = r != 0u ? 255 : 0;
-        }
-        v.mh[p] = v.raw[p] | hole;
-    }
+// unions across the right and bottom edges of every zone
+LV_D void zone_link_body(const View& v, uint32_t bid, uint32_t nblk) {
+    const int lane = (int)(threadIdx.x & 63u), wv = (int)(threadIdx.x >> 6);
+    const int nzx = (v.W + ZS - 1) / ZS, nzy = (v.H + ZS - 1) / ZS;
+    const int z = (int)bid * ZPB + wv;
+    if (z >= nzx * nzy) return;
+    const int zx = z % nzx, zy = z / nzx;
+    const uint8_t* e = v.zedge;
+    if (zx + 1 < nzx)
+        zone_link(v.zpar, (uint32_t)z, (uint32_t)z + 1u, e[(size_t)z * 256 + 192 + lane],
+                  e[(size_t)(z + 1) * 256 + 128 + lane], lane);
+    if (zy + 1 < nzy)
+        zone_link(v.zpar, (uint32_t)z, (uint32_t)(z + nzx), e[(size_t)z * 256 + 64 + lane],
+                  e[(size_t)(z + nzx) * 256 + lane], lane);
+}
+
+// holes of a zone: its 0-pixels minus the flood from the border pixels of outside components
+LV_D void zone_holes_body(const View& v, uint32_t bid, uint32_t nblk) {
+    const int lane = (int)(threadIdx.x & 63u), wv = (int)(threadIdx.x >> 6);
+    const int nzx = (v.W + ZS - 1) / ZS, nz = nzx * ((v.H + ZS - 1) / ZS);
+    const int z = (int)bid * ZPB + wv;
+    if (z >= nz) return;
+    const uint64_t zm = v.zbits[(size_t)z * ZS + lane];
+    const uint8_t* e = v.zedge + (size_t)z * 256;
+    const uint32_t cT = e[lane], cB = e[64 + lane], cL = e[128 + lane], cR = e[192 + lane];
+    const uint32_t k = v.zcount[z];
+    const uint32_t c00 = v.zedge[0];  // component of pixel (0,0) in zone 0, 255 if it is foreground
+    const uint32_t root = c00 == 255u ? 0xFFFFFFFFu : g_find_halving(v.zpar, c00);
+    const uint32_t base = (uint32_t)z * kZoneComps;
+    const bool o0 = (uint32_t)lane < k && g_find_halving(v.zpar, base + lane) == root;
+    const bool o1 = (uint32_t)lane + 64u < k && g_find_halving(v.zpar, base + lane + 64) == root;
+    const uint64_t m0 = __ballot(o0), m1 = __ballot(o1);
+    auto outside = [&](uint32_t c) {
+        return c < 64u ? ((m0 >> c) & 1ull) != 0 : (c < 128u && ((m1 >> (c - 64u)) & 1ull) != 0);
+    };
+    const uint64_t sT = __ballot(outside(cT)), sB = __ballot(outside(cB));
+    uint64_t s = (outside(cL) ? 1ull : 0ull) | (outside(cR) ? 1ull << 63 : 0ull);
+    if (lane == 0) s |= sT;
+    if (lane == 63) s |= sB;
+    const uint64_t o = flood(zm, brev64(zm), s, lane);
+    v.hbits[(size_t)z * ZS + lane] = zm & ~o;
 }
 
 // Post-processing masks are 0/255 bytes; the median and dilation kernels below work on four pixels
@@ -1543,6 +1646,16 @@ LV_D void store_mask_word(uint8_t* m, int W, int H, int gx, int gy, uint32_t w) 
     for (int i = 0; i < 4 && gx + i < W; ++i) row[gx + i] = (uint8_t)(w >> (8 * i));
 }
 
+// bits (gx..gx+3, gy) of a zone bitmap (gx % 4 == 0), zero outside the image
+LV_D uint32_t zone_bits4(const uint64_t* bits, int W, int H, int gx, int gy) {
+    if (gx < 0 || gy < 0 || gx >= W || gy >= H) return 0u;
+    const size_t zi = (size_t)(gy / ZS) * (size_t)((W + ZS - 1) / ZS) + (size_t)(gx / ZS);
+    return (uint32_t)(bits[zi * ZS + (size_t)(gy % ZS)] >> (gx % ZS)) & 15u;
+}
+// four bits -> four bytes of 0/255 (the products of the bits with 1, 2^7, 2^14, 2^21 never
+// overlap, so bit i lands alone at bit 8i)
+LV_D uint32_t expand4(uint32_t b) { return ((b * 0x00204081u) & 0x01010101u) * 255u; }
+
 // byte window of two adjacent words: bytes s..s+3 of (lo, hi)
 LV_D uint32_t bytes_at(uint32_t hi, uint32_t lo, uint32_t s) { return __builtin_amdgcn_alignbyte(hi, lo, s); }
 
@@ -1582,7 +1695,8 @@ LV_D void median_body(const View& v, uint32_t bid, uint32_t nblk) {
         // erosion treats the outside of the image as 255
         for (int i = threadIdx.x; i < EH * EWW; i += TPB) {
             const int ly = i / EWW, wx = i % EWW;
-            s_pre[ly][wx] = load_mask_word(v.pre, W, H, x0 - 12 + 4 * wx, y0 - HM - HE + ly, 255u);
+            // PreFlood = 255 except at its 0-pixels, 255 outside the image
+            s_pre[ly][wx] = expand4(~zone_bits4(v.zbits, W, H, x0 - 12 + 4 * wx, y0 - HM - HE + ly) & 15u);
         }
         __syncthreads();
         // horizontal 7-erosion; mask word j (columns x0-8+4j..) is PreFlood word j+1
@@ -1600,7 +1714,8 @@ LV_D void median_body(const View& v, uint32_t bid, uint32_t nblk) {
             uint32_t e = s_eh[ly][j];
 #pragma unroll
             for (int a = 1; a <= 2 * HE; ++a) e &= s_eh[ly + a][j];
-            m = (load_mask_word(v.mh, W, H, gx, gy, 0u) | e) & in_image_bytes(W, H, gx, gy);
+            m = (load_mask_word(v.raw, W, H, gx, gy, 0u) | expand4(zone_bits4(v.hbits, W, H, gx, gy)) | e) &
+                in_image_bytes(W, H, gx, gy);
         } else {
             m = load_mask_word(v.raw, W, H, gx, gy, 0u);
         }
@@ -2078,9 +2193,9 @@ template <int C, int ALGO, bool FUSED>
 __global__ void __launch_bounds__(TPB) k_update(View v) { update_body<C, ALGO, FUSED>(v, LV_SINGLE_BID); }
 template <int C>
 __global__ void __launch_bounds__(TPB) k_copy_last_color(View v) { copy_last_color_body<C>(v, LV_SINGLE_BID); }
-__global__ void __launch_bounds__(TPB) k_ccl_local(View v) { ccl_local_body(v, LV_SINGLE_BID); }
-__global__ void __launch_bounds__(TPB) k_ccl_border(View v) { ccl_border_body(v, LV_SINGLE_BID); }
-__global__ void __launch_bounds__(TPB) k_holes(View v) { holes_body(v, LV_SINGLE_BID); }
+__global__ void __launch_bounds__(TPB) k_zone_close(View v) { zone_close_body(v, LV_SINGLE_BID); }
+__global__ void __launch_bounds__(TPB) k_zone_link(View v) { zone_link_body(v, LV_SINGLE_BID); }
+__global__ void __launch_bounds__(TPB) k_zone_holes(View v) { zone_holes_body(v, LV_SINGLE_BID); }
 template <bool SUB>
 __global__ void __launch_bounds__(TPB) k_median(View v) { median_body<SUB>(v, LV_SINGLE_BID); }
 __global__ void __launch_bounds__(TPB) k_dilate_final(View v) { dilate_final_body(v, LV_SINGLE_BID); }
@@ -2122,9 +2237,9 @@ __global__ void __launch_bounds__(256) kb_fixpoint_tail(BatchArgs b, uint32_t k0
     });
 }
 __global__ void __launch_bounds__(TPB) kb_update(BatchArgs b) { LV_BATCH_C(update_body, kSuBSENSE, true); }
-__global__ void __launch_bounds__(TPB) kb_ccl_local(BatchArgs b) { for_batch_block(b, ccl_local_body); }
-__global__ void __launch_bounds__(TPB) kb_ccl_border(BatchArgs b) { for_batch_block(b, ccl_border_body); }
-__global__ void __launch_bounds__(TPB) kb_holes(BatchArgs b) { for_batch_block(b, holes_body); }
+__global__ void __launch_bounds__(TPB) kb_zone_close(BatchArgs b) { for_batch_block(b, zone_close_body); }
+__global__ void __launch_bounds__(TPB) kb_zone_link(BatchArgs b) { for_batch_block(b, zone_link_body); }
+__global__ void __launch_bounds__(TPB) kb_zone_holes(BatchArgs b) { for_batch_block(b, zone_holes_body); }
 __global__ void __launch_bounds__(TPB) kb_median(BatchArgs b) { for_batch_block(b, median_body<true>); }
 __global__ void __launch_bounds__(TPB) kb_dilate_final(BatchArgs b) { for_batch_block(b, dilate_final_body); }
 __global__ void __launch_bounds__(TPB) kb_downsample(BatchArgs b) { LV_BATCH_C(downsample_body); }
@@ -2142,11 +2257,9 @@ __global__ void __launch_bounds__(TPB) kb_refresh(BatchArgs b) {
 static inline unsigned cdiv(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }
 
 void kernel_grids(const View& v, const StepOptions& opt, uint32_t g[kNumGrids]) {
-    const unsigned ct = cdiv(v.W, CT) * cdiv(v.H, CT);
     g[kGridTile] = cdiv(v.W, TW) * cdiv(v.H, TH);
     g[kGridPost] = cdiv(v.W, PW) * cdiv(v.H, PH);
-    g[kGridCcl] = ct;
-    g[kGridCclBorder] = cdiv((size_t)ct * 2 * CT, TPB);
+    g[kGridZone] = cdiv((size_t)cdiv(v.W, ZS) * cdiv(v.H, ZS), ZPB);
     g[kGridPix] = cdiv(v.npx, TPB);
     g[kGridCell] = !v.lr_scaling ? 0u
                    : (v.W == 8 * v.dsW && v.H == 8 * v.dsH) ? cdiv(v.dsW, 32) * (unsigned)v.dsH
@@ -2180,9 +2293,9 @@ static void frame_step_t(const View& v, const StepOptions& opt, hipStream_t st) 
                        (uint32_t)opt.tail_cap);
     if constexpr (ALGO == kSuBSENSE) {
         hipLaunchKernelGGL((k_update<C, ALGO, true>), dim3(g[kGridTile]), dim3(TPB), 0, st, v);
-        hipLaunchKernelGGL(k_ccl_local, dim3(g[kGridCcl]), dim3(TPB), 0, st, v);
-        hipLaunchKernelGGL(k_ccl_border, dim3(g[kGridCclBorder]), dim3(TPB), 0, st, v);
-        hipLaunchKernelGGL(k_holes, dim3(g[kGridCcl]), dim3(TPB), 0, st, v);
+        hipLaunchKernelGGL(k_zone_close, dim3(g[kGridZone]), dim3(TPB), 0, st, v);
+        hipLaunchKernelGGL(k_zone_link, dim3(g[kGridZone]), dim3(TPB), 0, st, v);
+        hipLaunchKernelGGL(k_zone_holes, dim3(g[kGridZone]), dim3(TPB), 0, st, v);
         hipLaunchKernelGGL((k_median<true>), dim3(g[kGridPost]), dim3(TPB), 0, st, v);
         hipLaunchKernelGGL(k_dilate_final, dim3(g[kGridPost]), dim3(TPB), 0, st, v);
         if (g[kGridCell]) hipLaunchKernelGGL((k_downsample<C>), dim3(g[kGridCell]), dim3(TPB), 0, st, v);
@@ -2228,9 +2341,9 @@ void batch_step(const View* d_views, int ns, const uint32_t* const prefix[kNumGr
     hipLaunchKernelGGL(kb_fixpoint_tail, dim3(total[kGridOne]), dim3(256), 0, st, args(kGridOne),
                        (uint32_t)opt.fixpoint_launches + 1u, (uint32_t)opt.tail_cap);
     hipLaunchKernelGGL(kb_update, dim3(total[kGridTile]), dim3(TPB), 0, st, args(kGridTile));
-    hipLaunchKernelGGL(kb_ccl_local, dim3(total[kGridCcl]), dim3(TPB), 0, st, args(kGridCcl));
-    hipLaunchKernelGGL(kb_ccl_border, dim3(total[kGridCclBorder]), dim3(TPB), 0, st, args(kGridCclBorder));
-    hipLaunchKernelGGL(kb_holes, dim3(total[kGridCcl]), dim3(TPB), 0, st, args(kGridCcl));
+    hipLaunchKernelGGL(kb_zone_close, dim3(total[kGridZone]), dim3(TPB), 0, st, args(kGridZone));
+    hipLaunchKernelGGL(kb_zone_link, dim3(total[kGridZone]), dim3(TPB), 0, st, args(kGridZone));
+    hipLaunchKernelGGL(kb_zone_holes, dim3(total[kGridZone]), dim3(TPB), 0, st, args(kGridZone));
     hipLaunchKernelGGL(kb_median, dim3(total[kGridPost]), dim3(TPB), 0, st, args(kGridPost));
     hipLaunchKernelGGL(kb_dilate_final, dim3(total[kGridPost]), dim3(TPB), 0, st, args(kGridPost));
     if (total[kGridCell]) hipLaunchKernelGGL(kb_downsample, dim3(total[kGridCell]), dim3(TPB), 0, st, args(kGridCell));
@@ -2266,6 +2379,16 @@ void launch_lbsp_abs(const uint8_t* img, int W, int H, int C, uint32_t t, uint16
     const unsigned tiles = cdiv(W, TW) * cdiv(H, TH);
     if (C == 3) hipLaunchKernelGGL(k_lbsp_abs<3>, dim3(tiles), dim3(TPB), 0, st, img, W, H, t, out);
     else hipLaunchKernelGGL(k_lbsp_abs<1>, dim3(tiles), dim3(TPB), 0, st, img, W, H, t, out);
+}
+
+// the SuBSENSE hole-filling and median kernels alone, on whatever v.raw holds
+void launch_postprocess_probe(const View& v, hipStream_t st) {
+    uint32_t g[kNumGrids];
+    kernel_grids(v, StepOptions{}, g);
+    hipLaunchKernelGGL(k_zone_close, dim3(g[kGridZone]), dim3(TPB), 0, st, v);
+    hipLaunchKernelGGL(k_zone_link, dim3(g[kGridZone]), dim3(TPB), 0, st, v);
+    hipLaunchKernelGGL(k_zone_holes, dim3(g[kGridZone]), dim3(TPB), 0, st, v);
+    hipLaunchKernelGGL((k_median<true>), dim3(g[kGridPost]), dim3(TPB), 0, st, v);
 }
 
 void launch_rng_probe(const View& v, uint32_t frame, uint32_t phase, uint32_t ev, uint32_t n, uint32_t* out,

@@ -26,9 +26,8 @@ struct StepOptions {
 // grid kinds of the frame-step kernels (block counts per stream)
 enum GridKind : int {
     kGridTile = 0,   // 64x4 pixel tiles (describe, match, update)
-    kGridPost,       // 64x16 tiles (close, median, dilate)
-    kGridCcl,        // 32x32 tiles (local labelling)
-    kGridCclBorder,  // tile-border unions
+    kGridPost,       // 64x16 tiles (median, dilate)
+    kGridZone,       // 64x64 hole-filling zones, four per block
     kGridPix,        // 256 pixels per block
     kGridCell,       // 256 frame-level cells per block (0 when learning-rate scaling is off)
     kGridOne,        // one block per stream
@@ -49,6 +48,7 @@ void launch_refresh(const View& v, float frac, int force_fg, uint32_t frame, uin
 void launch_init_desc(const View& v, const uint8_t* img, hipStream_t st);
 void launch_background(const View& v, uint8_t* out_img, uint16_t* out_desc, hipStream_t st);
 void launch_lbsp_abs(const uint8_t* img, int W, int H, int C, uint32_t t, uint16_t* out, hipStream_t st);
+void launch_postprocess_probe(const View& v, hipStream_t st);
 void launch_rng_probe(const View& v, uint32_t frame, uint32_t phase, uint32_t ev, uint32_t n, uint32_t* out,
                       hipStream_t st);
 

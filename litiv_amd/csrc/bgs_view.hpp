@@ -36,8 +36,13 @@ struct View {
     // u8 W*H maps
     uint8_t *unstable, *blinks, *last_raw, *last_raw_blink, *last_fg, *last_dil_inv;
     // per-frame scratch
-    uint8_t *raw, *pre, *mh;
-    uint32_t* label;
+    uint8_t* raw;            // W*H raw segmentation (0/255)
+    // hole filling on 64x64 zones (row-major zone grid): bit x of word 64*z+y is pixel (x, y) of
+    // zone z; zbits = zero pixels of the closed mask, hbits = holes
+    uint64_t *zbits, *hbits;
+    uint8_t* zedge;          // 256 per zone: border component of each zone-border pixel, 255 = none
+    uint32_t* zpar;          // 128 per zone: union-find parents of the zones' border components
+    uint32_t* zcount;        // per zone: number of border components
     uint16_t* desc_cur;      // W*H*C intra descriptors of the current frame
     uint32_t* rec;           // W*H update records
     uint32_t* mres;          // W*H match results
