@@ -160,7 +160,7 @@ def cpu_baseline(algo, classes, lr, budget_s):
 def run_batch(args, preset, n_streams, ref_frames):
     import torch
     from litiv_amd import BackgroundSubtractorSuBSENSE, Batch, synth
-    from litiv_amd.autotune import tune_batch
+    from litiv_amd.autotune import OnlineTuner
 
     steps = args.steps if args.steps is not None else ref_frames - args.warmup
     total = args.warmup + steps
@@ -215,20 +215,28 @@ def run_batch(args, preset, n_streams, ref_frames):
         plain = px_step * steps / wall_plain / 1e6
         log("plain serial: %.3f ms/step, %.1f Mpix/s" % (wall_plain / steps * 1e3, plain))
 
-    # (2) the batch: warm-up steps double as the autotuner's trials (options never change results)
+    # (2) the batch, stepped through the online tuner: a trial round of the launch-option candidates
+    # every 64 steps, warm-up and timed steps alike (options never change results)
     init_all()
     batch = Batch(subs)
-    tuned = tune_batch(batch, lambda t: ptrs(t), args.warmup, log=log) if not args.no_tune else batch.options()
+    tuner = None if args.no_tune else OnlineTuner(batch, period=64, log=log)
+    step = tuner.apply if tuner else batch.apply_device
+    for t in range(args.warmup):
+        step(*ptrs(t))
     batch.synchronize()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for k in range(steps):
-        batch.apply_device(*ptrs(args.warmup + k))
+        step(*ptrs(args.warmup + k))
     batch.synchronize()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     mpix = px_step * steps / wall / 1e6
+    tuned = batch.options()
     log("batched: %.3f ms/step, %.1f Mpix/s (options %s)" % (wall / steps * 1e3, mpix, tuned))
+    tune_info = None if tuner is None else {"rounds": tuner.rounds, "period": tuner.period,
+                                            "candidates": tuner.cands,
+                                            "median_step_ms": [round(m, 4) if m else m for m in tuner.medians()]}
     batch.close()
 
     # (3) measurement passes over the same frames (re-initialized): match-kernel time with HIP events
@@ -289,6 +297,7 @@ def run_batch(args, preset, n_streams, ref_frames):
                    "mix": "16x320x240x3, 16x640x480x3, 16x640x512x1, 16x1920x1080x3",
                    "pixels_per_step": px_step, "frames": total, "algo": "subsense", "learning_rate": 0.0,
                    "launch": "one launch per kernel for all streams, hipGraph replay", "options": tuned,
+                   "autotune": tune_info,
                    "parallelism": "64 streams batched on one GPU"},
         "plain_serial": None if plain is None else {
             "value": round(plain, 2), "unit": "Mpix/s",
@@ -308,9 +317,11 @@ def run_batch(args, preset, n_streams, ref_frames):
                      "mean_worklist_per_stream": [round(float(x), 1) for x in (work / len(iters))[1:6]]},
     }
     if args.serve_seconds > 0:
-        log("serving: %d stream slots, %.0f s warm-up + %.0f s measured" % (args.serve_streams, args.serve_warmup,
-                                                                            args.serve_seconds))
-        result["serving"] = run_serve(args)
+        result["serving"] = {}
+        for n in [int(x) for x in str(args.serve_streams).split(",") if x]:
+            log("serving: %d stream slots, %.0f s warm-up + %.0f s measured" % (n, args.serve_warmup,
+                                                                                args.serve_seconds))
+            result["serving"]["%d_streams" % n] = run_serve(args, n)
         log("serving: %s" % json.dumps(result["serving"]))
     if not args.no_cpu:
         # streams 16g .. 16g+threads-1 of each resolution class g, one per thread, first frames
@@ -331,8 +342,8 @@ def run_batch(args, preset, n_streams, ref_frames):
 # serving: streams arrive and leave, frames arrive as Poisson processes (wall clock), the server
 # batches whatever is waiting
 # ------------------------------------------------------------------------------------------------
-def run_serve(args, log_prefix="serve"):
-    """256 stream slots; resolutions from the SERVE generator preset ({320x240, 640x480, 1280x720,
+def run_serve(args, n_slots, log_prefix="serve"):
+    """n_slots stream slots; resolutions from the SERVE generator preset ({320x240, 640x480, 1280x720,
     1920x1080} RGB with p = .3/.3/.2/.2); frames arrive at 30 fps (Poisson) per stream; lifetimes are
     exponential with a mean of 2000 frames and an ended stream is replaced by a new one; models are
     admitted against a 250 GB budget (streams that do not fit wait). Latency = frame arrival ->
@@ -342,10 +353,10 @@ def run_serve(args, log_prefix="serve"):
     import torch
     from litiv_amd import Server, synth
 
-    n_slots, fps, mean_life, ring = args.serve_streams, 30.0, 2000.0, 16
+    fps, mean_life, ring = 30.0, 2000.0, 16
     rng = np.random.default_rng(0x5E12E)
     srv = Server(hbm_budget=250 << 30, max_inflight=2, max_batch=n_slots, fix_px_per_block=8192,
-                 cont_px_per_block=2048)
+                 cont_px_per_block=2048, seq_limit=8)
     dev = torch.device("cuda:0")
     incarnation = [0]
     slots = {}      # server stream id -> dict
@@ -595,7 +606,7 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=20.0,
                     help="batched workload: measured seconds of the serving run (0 = skip)")
     ap.add_argument("--serve-warmup", type=float, default=5.0)
-    ap.add_argument("--serve-streams", type=int, default=256)
+    ap.add_argument("--serve-streams", default="256,200", help="comma-separated stream-slot counts, one run each")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; multi-GPU runs (--gpus %d) are not part of it. Streams "
