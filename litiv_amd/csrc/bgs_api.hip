@@ -880,13 +880,14 @@ extern "C" void* lvbgs_batch_stream(lvbgs_batch* b) { return b ? (void*)b->strea
 extern "C" int lvbgs_batch_default_options(lvbgs_batch_options* o) {
     if (!o) return LVBGS_ERR_ARG;
     o->use_graph = 1;
-    o->seq_limit = 255;
+    // what the online tuner settles on for settled models of mixed resolutions
+    o->seq_limit = 12;
     o->cont_blocks = 1024;
     o->cont_px_per_block = 2048;
-    o->fixpoint_launches = 3;
+    o->fixpoint_launches = 2;
     o->fixpoint_blocks = 256;
     o->fix_px_per_block = 8192;
-    o->fetch_depth = 2;
+    o->fetch_depth = 4;
     return LVBGS_OK;
 }
 
