@@ -441,8 +441,10 @@ LV_D void raw_current(const View& v, const uint8_t* frame, uint32_t q, uint64_t&
 
 // Fills the thread's entries from the applied writers, the latest in raster order first (it owns
 // its slot). Returns the slot bitmap (mod 64) and the entry count, or false beyond kRepl slots.
-template <int C>
-LV_D bool build_repl(const View& v, const uint8_t* frame, int x, int y, uint32_t applied, uint64_t& sbm, int& n) {
+// rec_of(i) is the update record of predecessor i.
+template <int C, typename RecOf>
+LV_D bool build_repl(const View& v, const uint8_t* frame, int x, int y, uint32_t applied, uint64_t& sbm, int& n,
+                     RecOf rec_of) {
     ReplLds& R = repl_lds();
     const int t = (int)threadIdx.x;
     sbm = 0;
@@ -451,7 +453,7 @@ LV_D bool build_repl(const View& v, const uint8_t* frame, int x, int y, uint32_t
         const int i = 31 - __builtin_clz(m);
         m &= ~(1u << i);
         const uint32_t q = (uint32_t)((y + kPredDY[i]) * v.W + (x + kPredDX[i]));
-        const uint32_t sl = rec_slot_diff(v.rec[q]);
+        const uint32_t sl = rec_slot_diff(rec_of(i));
         bool seen = false;
         for (int k = 0; k < n; ++k) seen = seen || R.slot[k][t] == sl;
         if (seen) continue;
@@ -469,16 +471,50 @@ LV_D bool build_repl(const View& v, const uint8_t* frame, int x, int y, uint32_t
     return true;
 }
 
+// One batch of KF loaded samples of a sequential scan (stops at the limit or at REQ matches).
+template <int C, int ALGO, int RS, int KF>
+LV_D void scan_batch(const uint8_t* lut, const PxCtx<C>& px, uint64_t sbm, int nrep, ScanState& st, uint32_t limit,
+                     uint32_t REQ, const uint64_t (&ra)[KF], const uint32_t (&rb)[KF]) {
+    const ReplLds& R = repl_lds();
+    const int t = (int)threadIdx.x;
+#pragma unroll
+    for (int j = 0; j < KF; ++j) {
+        if (st.s >= limit || st.good >= REQ) break;
+        uint64_t a = ra[j];
+        uint32_t b = rb[j];
+        if ((sbm >> (st.s & 63u)) & 1u) {
+            for (int k = 0; k < nrep; ++k)
+                if (R.slot[k][t] == st.s) {
+                    a = (uint64_t)R.w0[k][t] | ((uint64_t)R.w1[k][t] << 32);
+                    b = R.w2[k][t];
+                    break;
+                }
+        }
+        uint32_t col[C], desc[C];
+        decode_raw<C>(a, b, col, desc);
+        uint32_t td = 0, ts = 0;
+        if (test_sample<C, ALGO, RS>(px, lut, col, desc, td, ts)) {
+            st.md = td < st.md ? td : st.md;
+            st.ms = ts < st.ms ? ts : st.ms;
+            ++st.good;
+        }
+        ++st.s;
+    }
+}
+
+// words between the records of two consecutive samples of a pixel
+template <int C>
+constexpr uint32_t kSampleStride = 64u * (C == 3 ? 3u : 1u);
+
+// Sequential scan from st.s up to the limit. Unconditional loads (the sample planes are padded by
+// kMaxFetch samples) at constant offsets from a running pointer keep all KF loads in flight together
+// without address arithmetic.
 template <int C, int ALGO, int RS, int KF>
 LV_D void scan_seq(const View& v, const uint8_t* lut, const PxCtx<C>& px, int x, int y, uint64_t sbm, int nrep,
                    ScanState& st, uint32_t limit) {
     const uint32_t p = (uint32_t)(y * v.W + x);
     const uint32_t REQ = (uint32_t)v.REQ;
-    const ReplLds& R = repl_lds();
-    const int t = (int)threadIdx.x;
-    constexpr uint32_t kStride = 64u * (C == 3 ? 3u : 1u);  // words between samples of a pixel
-    // Unconditional loads (the sample planes are padded by kMaxFetch samples) at constant offsets
-    // from a running pointer keep all KF loads in flight together without address arithmetic.
+    constexpr uint32_t kStride = kSampleStride<C>;
     const uint32_t* cur = rec_base(v, p) + st.s * kStride;
     while (st.s < limit && st.good < REQ) {
         uint64_t ra[KF];
@@ -486,29 +522,7 @@ LV_D void scan_seq(const View& v, const uint8_t* lut, const PxCtx<C>& px, int x,
 #pragma unroll
         for (int j = 0; j < KF; ++j) fetch_at<C>(cur + j * kStride, ra[j], rb[j]);
         cur += KF * kStride;
-#pragma unroll
-        for (int j = 0; j < KF; ++j) {
-            if (st.s >= limit || st.good >= REQ) break;
-            uint64_t a = ra[j];
-            uint32_t b = rb[j];
-            if ((sbm >> (st.s & 63u)) & 1u) {
-                for (int k = 0; k < nrep; ++k)
-                    if (R.slot[k][t] == st.s) {
-                        a = (uint64_t)R.w0[k][t] | ((uint64_t)R.w1[k][t] << 32);
-                        b = R.w2[k][t];
-                        break;
-                    }
-            }
-            uint32_t col[C], desc[C];
-            decode_raw<C>(a, b, col, desc);
-            uint32_t td = 0, ts = 0;
-            if (test_sample<C, ALGO, RS>(px, lut, col, desc, td, ts)) {
-                st.md = td < st.md ? td : st.md;
-                st.ms = ts < st.ms ? ts : st.ms;
-                ++st.good;
-            }
-            ++st.s;
-        }
+        scan_batch<C, ALGO, RS, KF>(lut, px, sbm, nrep, st, limit, REQ, ra, rb);
     }
 }
 
@@ -626,14 +640,25 @@ LV_D void describe_body(const View& v, uint32_t bid, uint32_t nblk) {
         v.sc->spec_part[threadIdx.x] = 0;
         v.sc->coop_part[threadIdx.x] = 0;
     }
+    const int lx = threadIdx.x % TW, ly = threadIdx.x / TW, x = x0 + lx, y = y0 + ly;
+    const bool inside = x < v.W && y < v.H;
+    // the pixel's own state is loaded first, so these loads run alongside the tile's
+    const uint32_t pp = inside ? (uint32_t)(y * v.W + x) : 0u;
+    const uint8_t roi_p = v.roi[pp];
+    float Rv = 0.f, Tv = 0.f, d_lt = 0.f, d_st = 0.f;
+    if constexpr (ALGO == kSuBSENSE) {
+        Rv = v.R[pp];
+        Tv = v.T[pp];
+        d_lt = v.RawLT[pp] - v.FinLT[pp];
+        d_st = v.RawST[pp] - v.FinST[pp];
+    }
     load_frame_tile<C>(tile, frame, v.W, v.H, x0, y0);
     load_lut(lut, v.sc);
     __syncthreads();
-    const int lx = threadIdx.x % TW, ly = threadIdx.x / TW, x = x0 + lx, y = y0 + ly;
     bool nz = false;
-    if (x < v.W && y < v.H) {
-        const uint32_t p = (uint32_t)(y * v.W + x);
-        if (!v.roi[p]) {
+    if (inside) {
+        const uint32_t p = pp;
+        if (!roi_p) {
             v.rec[p] = kRecNone;
         } else {
             uint32_t pc = 0;
@@ -660,10 +685,8 @@ LV_D void describe_body(const View& v, uint32_t bid, uint32_t nblk) {
             };
             uint32_t r;
             if constexpr (ALGO == kSuBSENSE) {
-                const float Rv = v.R[p];
-                const uint32_t unst_new =
-                    (Rv > 3.0f || (v.RawLT[p] - v.FinLT[p]) > 0.1f || (v.RawST[p] - v.FinST[p]) > 0.1f) ? 1u : 0u;
-                const uint64_t L = isinf(lr) ? ~0ull : (lr > 0 ? (uint64_t)ceil(lr) : (uint64_t)ceilf(v.T[p]));
+                const uint32_t unst_new = (Rv > 3.0f || d_lt > 0.1f || d_st > 0.1f) ? 1u : 0u;
+                const uint64_t L = isinf(lr) ? ~0ull : (lr > 0 ? (uint64_t)ceil(lr) : (uint64_t)ceilf(Tv));
                 const bool self_bg = mod64(d[0], L) == 0u;
                 const bool self_fg = v.sc->cooldown != 0u && (d[0] % 2u) == 0u;
                 const bool use3 = v.use3x3 && !unst_new;
@@ -724,6 +747,55 @@ LV_D void build_ctx(const View& v, const uint8_t* frame, const volatile uint8_t*
     else lobster_thresholds<C>(v, px);
 }
 
+// a tile pixel's per-frame match inputs, loaded together with the tile and staged in LDS (the
+// regrouping of a tile's pixels moves them between threads after the tile barrier)
+template <int C>
+struct PxIn {
+    uint16_t desc[C];  // intra descriptors of the current frame
+    uint8_t roi, unst;
+    float R;
+};
+
+template <int C, int ALGO>
+LV_D PxIn<C> load_px_in(const View& v, uint32_t q, bool in) {
+    PxIn<C> r;
+    r.roi = in ? v.roi[q] : (uint8_t)0;
+#pragma unroll
+    for (int c = 0; c < C; ++c) r.desc[c] = v.desc_cur[(size_t)q * C + c];
+    if constexpr (ALGO == kSuBSENSE) {
+        r.R = v.R[q];
+        r.unst = v.unstable[q];
+    } else {
+        r.R = 0.f;
+        r.unst = 0;
+    }
+    return r;
+}
+
+// the match context from the LDS frame tile (5x5 window at win) and the staged inputs
+template <int C, int ALGO, int RS>
+LV_D void build_ctx_staged(const View& v, const volatile uint8_t* win, const PxIn<C>& in, PxCtx<C>& px) {
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+        px.cur[c] = win[2 * RS + 2 * C + c];
+        px.intra[c] = in.desc[c];
+    }
+#pragma unroll
+    for (int c = 0; c < C; ++c)
+#pragma unroll
+        for (int w = 0; w < 4; ++w) {
+            uint32_t acc = 0;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const int n = 4 * w + k;
+                acc |= (uint32_t)win[(kLbspDY[n] + 2) * RS + (kLbspDX[n] + 2) * C + c] << (8 * k);
+            }
+            px.taps[c][w] = acc;
+        }
+    if constexpr (ALGO == kSuBSENSE) subsense_thresholds<C>(v, in.R, in.unst, px);
+    else lobster_thresholds<C>(v, px);
+}
+
 // the same context with the taps read from the frame in global memory
 template <int C, int ALGO>
 LV_D void build_ctx_global(const View& v, const uint8_t* frame, int x, int y, PxCtx<C>& px) {
@@ -764,8 +836,8 @@ LV_D void stage_patch(const uint8_t* frame, int W, int x, int y, uint8_t* patch,
 struct PredTile;
 template <int C, int ALGO, int KF>
 LV_D bool match_pixel(const View& v, const uint8_t* frame, const uint8_t* lut, const FrameTile<C>& tile,
-                      const PredTile& pred, int lx, int ly, int x, int y, uint32_t p, uint32_t seq_limit,
-                      uint32_t& examined);
+                      const PredTile& pred, const PxIn<C>& in, int lx, int ly, int x, int y, uint32_t p,
+                      uint32_t seq_limit, uint32_t& examined);
 
 // records and previous raw decisions of the raster-earlier part of a tile's 5x5 neighbourhood:
 // rows y0-2 .. y0+TH-1, columns x0-2 .. x0+TW+1
@@ -810,17 +882,22 @@ LV_D void match_body(const View& v, uint32_t bid, uint32_t nblk) {
     const int x0 = (tid % ntx) * TW, y0 = (tid / ntx) * TH;
     __shared__ uint16_t s_perm[TPB];
     __shared__ uint32_t s_nfg[TPB / 64];
+    __shared__ PxIn<C> s_in[TPB];
+    // this thread's tile pixel before the regrouping: its inputs are loaded alongside the tile
+    const int ox = x0 + (int)threadIdx.x % TW, oy = y0 + (int)threadIdx.x / TW;
+    const bool oin = ox < v.W && oy < v.H;
+    const PxIn<C> pin = load_px_in<C, ALGO>(v, oin ? (uint32_t)(oy * v.W + ox) : 0u, oin);
     load_frame_tile<C>(tile, frame, v.W, v.H, x0, y0);
     load_pred_tile(pred, v, x0, y0);
     load_lut(lut, v.sc);
+    s_in[threadIdx.x] = pin;
     __syncthreads();
     // A wavefront scans as long as its slowest lane, and foreground pixels scan every sample: the
     // tile's pixels are regrouped so those that were foreground in the previous frame share
     // wavefronts (stable order within each group). Each pixel's match depends on the tile data only.
     {
-        const int t = (int)threadIdx.x, ox = x0 + t % TW, oy = y0 + t / TW;
-        const bool likely_fg = ox < v.W && oy < v.H && pred.guess[t / TW + 2][t % TW + 2] != 0 &&
-                               v.roi[(size_t)oy * v.W + ox];
+        const int t = (int)threadIdx.x;
+        const bool likely_fg = oin && pred.guess[t / TW + 2][t % TW + 2] != 0 && pin.roi;
         const unsigned long long bal = __ballot(likely_fg);
         const int lane = t & 63, wave = t >> 6;
         if (lane == 0) s_nfg[wave] = (uint32_t)__popcll(bal);
@@ -839,8 +916,9 @@ LV_D void match_body(const View& v, uint32_t bid, uint32_t nblk) {
     const uint32_t p = (uint32_t)(y * v.W + x);
     uint32_t examined = 0;
     bool cont = false;
-    if (x < v.W && y < v.H && v.roi[p])
-        cont = !match_pixel<C, ALGO, KF>(v, frame, lut, tile, pred, lx, ly, x, y, p, seq_limit, examined);
+    const PxIn<C> in = s_in[src];
+    if (x < v.W && y < v.H && in.roi)
+        cont = !match_pixel<C, ALGO, KF>(v, frame, lut, tile, pred, in, lx, ly, x, y, p, seq_limit, examined);
     // unfinished pixels go to the cooperative phase (one atomic per wavefront)
     const unsigned long long bal = __ballot(cont);
     if (bal) {
@@ -857,10 +935,10 @@ LV_D void match_body(const View& v, uint32_t bid, uint32_t nblk) {
 // otherwise its partial state is left in mres for the cooperative phase.
 template <int C, int ALGO, int KF>
 LV_D bool match_pixel(const View& v, const uint8_t* frame, const uint8_t* lut, const FrameTile<C>& tile,
-                      const PredTile& pred, int lx, int ly, int x, int y, uint32_t p, uint32_t seq_limit,
-                      uint32_t& examined) {
+                      const PredTile& pred, const PxIn<C>& in, int lx, int ly, int x, int y, uint32_t p,
+                      uint32_t seq_limit, uint32_t& examined) {
     PxCtx<C> px;
-    build_ctx<C, ALGO, FrameTile<C>::LW * C>(v, frame, &tile.px[(ly * FrameTile<C>::LW + lx) * C], x, y, px);
+    build_ctx_staged<C, ALGO, FrameTile<C>::LW * C>(v, &tile.px[(ly * FrameTile<C>::LW + lx) * C], in, px);
     // guess: raster-earlier writers decide as in the previous frame
     const uint32_t applied = applied_writers_tile(pred, lx, ly);
     uint64_t sbm;
@@ -868,7 +946,9 @@ LV_D bool match_pixel(const View& v, const uint8_t* frame, const uint8_t* lut, c
     ScanState st = scan_init<C>();
     const uint32_t N = (uint32_t)v.N;
     // more than kRepl overwritten slots (very rare): the whole scan goes to the cooperative phase
-    const uint32_t limit = build_repl<C>(v, frame, x, y, applied, sbm, nrep) ? (seq_limit < N ? seq_limit : N) : 0u;
+    auto rec_of = [&](int i) { return pred.rec[ly + 2 + kPredDY[i]][lx + 2 + kPredDX[i]]; };
+    const uint32_t limit =
+        build_repl<C>(v, frame, x, y, applied, sbm, nrep, rec_of) ? (seq_limit < N ? seq_limit : N) : 0u;
     scan_seq<C, ALGO, FrameTile<C>::LW * C, KF>(v, lut, px, x, y, sbm, nrep, st, limit);
     examined = st.s;
     v.mres[p] = pack_state(st);
@@ -931,7 +1011,8 @@ LV_D void match_seq_body(const View& v, uint32_t bid, uint32_t nblk) {
         const uint32_t applied = applied_writers_global<false>(v, x, y);
         uint64_t sbm;
         int nrep;
-        if (!build_repl<C>(v, frame, x, y, applied, sbm, nrep)) {
+        auto rec_of = [&](int i) { return v.rec[(size_t)(y + kPredDY[i]) * v.W + (x + kPredDX[i])]; };
+        if (!build_repl<C>(v, frame, x, y, applied, sbm, nrep, rec_of)) {
             v.cont2[atomicAdd(&v.sc->cont2_count, 1u)] = p;
             continue;
         }
