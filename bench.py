@@ -215,11 +215,12 @@ def run_batch(args, preset, n_streams, ref_frames):
         plain = px_step * steps / wall_plain / 1e6
         log("plain serial: %.3f ms/step, %.1f Mpix/s" % (wall_plain / steps * 1e3, plain))
 
-    # (2) the batch, stepped through the online tuner: a trial round of the launch-option candidates
-    # every 64 steps, warm-up and timed steps alike (options never change results)
+    # (2) the batch, stepped through the online tuner: trial rounds (shared options, or one resolution
+    # class's launch shape) every 32 steps, fewer once the choices settle; warm-up and timed steps
+    # alike (options never change results)
     init_all()
     batch = Batch(subs)
-    tuner = None if args.no_tune else OnlineTuner(batch, period=64, log=log)
+    tuner = None if args.no_tune else OnlineTuner(batch, log=log)
     step = tuner.apply if tuner else batch.apply_device
     for t in range(args.warmup):
         step(*ptrs(t))
@@ -235,8 +236,10 @@ def run_batch(args, preset, n_streams, ref_frames):
     tuned = batch.options()
     log("batched: %.3f ms/step, %.1f Mpix/s (options %s)" % (wall / steps * 1e3, mpix, tuned))
     tune_info = None if tuner is None else {"rounds": tuner.rounds, "period": tuner.period,
-                                            "candidates": tuner.cands,
-                                            "median_step_ms": [round(m, 4) if m else m for m in tuner.medians()]}
+                                            "shared_candidates": tuner.shared,
+                                            "shared_median_step_ms": [round(m, 4) if m else m
+                                                                      for m in tuner.shared_medians()],
+                                            "per_resolution": tuner.class_choices()}
     batch.close()
 
     # (3) measurement passes over the same frames (re-initialized): match-kernel time with HIP events
@@ -245,6 +248,8 @@ def run_batch(args, preset, n_streams, ref_frames):
     log("measurement pass: match-kernel timing and counters over %d steps" % n_meas)
     init_all()
     batch = Batch(subs, **tuned)
+    if tuner is not None:
+        tuner.apply_choices(batch)
     for t in range(args.warmup):
         batch.apply_device(*ptrs(t))
     batch.set_timing("match")
@@ -266,6 +271,8 @@ def run_batch(args, preset, n_streams, ref_frames):
     log("measurement pass: step latency over %d steps" % n_meas)
     init_all()
     batch = Batch(subs, **tuned)
+    if tuner is not None:
+        tuner.apply_choices(batch)
     for t in range(args.warmup):
         batch.apply_device(*ptrs(t))
     batch.set_timing("step")

@@ -188,6 +188,13 @@ typedef struct lvbgs_batch_options {
 int lvbgs_batch_default_options(lvbgs_batch_options* out);
 int lvbgs_batch_get_options(const lvbgs_batch* b, lvbgs_batch_options* out);
 int lvbgs_batch_set_options(lvbgs_batch* b, const lvbgs_batch_options* opts);
+/* Per-stream overrides of seq_limit, cont_px_per_block and fix_px_per_block for member `index`
+ * (0 = the batch's value), so the launch shape can follow each stream's resolution. Results never
+ * depend on them. */
+int lvbgs_batch_set_stream_options(lvbgs_batch* b, int index, int seq_limit, int cont_px_per_block,
+                                   int fix_px_per_block);
+int lvbgs_batch_get_stream_options(const lvbgs_batch* b, int index, int* seq_limit, int* cont_px_per_block,
+                                   int* fix_px_per_block);
 /* one apply() of every stream: d_imgs[i] is stream i's frame, d_masks (nullable) its output mask,
  * lrs (nullable = 0) its learning-rate override */
 int lvbgs_batch_apply_device(lvbgs_batch* b, const void* const* d_imgs, void* const* d_masks, const double* lrs);

@@ -152,6 +152,8 @@ def lib():
         L.lvbgs_batch_default_options.argtypes = [C.POINTER(BatchOptions)]
         L.lvbgs_batch_get_options.argtypes = [vp, C.POINTER(BatchOptions)]
         L.lvbgs_batch_set_options.argtypes = [vp, C.POINTER(BatchOptions)]
+        L.lvbgs_batch_set_stream_options.argtypes = [vp, i, i, i, i]
+        L.lvbgs_batch_get_stream_options.argtypes = [vp, i, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
         L.lvbgs_batch_apply_device.argtypes = [vp, C.POINTER(vp), C.POINTER(vp), C.POINTER(C.c_double)]
         L.lvbgs_batch_synchronize.argtypes = [vp]
         L.lvbgs_batch_set_timing.argtypes = [vp, C.c_char_p]
@@ -428,6 +430,31 @@ class Batch:
                 raise TypeError("unknown batch option %r" % k)
             setattr(o, k, int(v))
         self._check(lib().lvbgs_batch_set_options(self._h, C.byref(o)))
+
+    STREAM_OPTIONS = ("seq_limit", "cont_px_per_block", "fix_px_per_block")
+
+    def stream_options(self, index):
+        """Member `index`'s overrides of the batch options (0 = the batch's value)."""
+        vals = [C.c_int() for _ in self.STREAM_OPTIONS]
+        self._check(lib().lvbgs_batch_get_stream_options(self._h, int(index), *[C.byref(x) for x in vals]))
+        return {k: x.value for k, x in zip(self.STREAM_OPTIONS, vals)}
+
+    def set_stream_options(self, index, **kw):
+        """Overrides for member `index` (seq_limit, cont_px_per_block, fix_px_per_block; 0 = the batch's
+        value); unspecified ones keep their current override."""
+        cur = self.stream_options(index)
+        for k, v in kw.items():
+            if k not in cur:
+                raise TypeError("unknown stream option %r" % k)
+            cur[k] = int(v)
+        self._check(lib().lvbgs_batch_set_stream_options(self._h, int(index), *[cur[k] for k in self.STREAM_OPTIONS]))
+
+    def shape_classes(self):
+        """Member indices grouped by (width, height, channels), largest total pixel count first."""
+        groups = {}
+        for i, m in enumerate(self.members):
+            groups.setdefault(m.shape, []).append(i)
+        return sorted(groups.items(), key=lambda kv: -kv[0][0] * kv[0][1] * len(kv[1]))
 
     def apply_device(self, d_imgs, d_masks=None, learningRates=None):
         """One apply() of every member: device frame pointers (and optional mask pointers) in member order."""

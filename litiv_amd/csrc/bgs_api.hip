@@ -769,6 +769,12 @@ struct lvbgs_batch {
     const uint32_t* prefix[kNumGrids] = {};
     uint32_t total[kNumGrids] = {};
     StepOptions opt{};
+    // per-stream overrides of the batch options (0 = the batch's value)
+    struct StreamOverride {
+        int seq_limit = 0, cont_px_per_block = 0, fix_px_per_block = 0;
+    };
+    std::vector<StreamOverride> sopt;
+    bool dirty = false;  // options changed since the grids were planned: re-plan at the next step
     bool any_cont = false;
     bool use_graph = false;
     hipGraphExec_t graph = nullptr;
@@ -798,15 +804,26 @@ static void batch_release(lvbgs_batch* b) {
     if (b->stream) hipStreamDestroy(b->stream);
 }
 
+// the options of stream i: the batch's, with the stream's overrides
+static StepOptions stream_options(const lvbgs_batch* b, int i) {
+    StepOptions o = b->opt;
+    const auto& so = b->sopt[(size_t)i];
+    if (so.seq_limit) o.seq_limit = so.seq_limit;
+    if (so.cont_px_per_block) o.cont_px_per_block = so.cont_px_per_block;
+    if (so.fix_px_per_block) o.fix_px_per_block = so.fix_px_per_block;
+    return o;
+}
+
 static int batch_plan(lvbgs_batch* b) {
     const int ns = (int)b->ctx.size();
     std::vector<uint32_t> pre((size_t)kNumGrids * (ns + 1), 0);
     b->any_cont = false;
     for (int i = 0; i < ns; ++i) {
         uint32_t g[kNumGrids];
-        kernel_grids(b->ctx[i]->v, b->opt, g);
+        const StepOptions o = stream_options(b, i);
+        kernel_grids(b->ctx[i]->v, o, g);
         for (int k = 0; k < kNumGrids; ++k) pre[(size_t)k * (ns + 1) + i + 1] = pre[(size_t)k * (ns + 1) + i] + g[k];
-        if (b->opt.seq_limit < b->ctx[i]->v.N) b->any_cont = true;
+        if (o.seq_limit < b->ctx[i]->v.N) b->any_cont = true;
     }
     if (!b->d_prefix && hipMalloc(&b->d_prefix, pre.size() * sizeof(uint32_t)) != hipSuccess)
         return bset_err(b, LVBGS_ERR_ALLOC, "batch tables");
@@ -832,6 +849,7 @@ extern "C" int lvbgs_batch_create(lvbgs_ctx* const* ctxs, int n, lvbgs_batch** o
     }
     lvbgs_batch* b = new lvbgs_batch();
     b->ctx.assign(ctxs, ctxs + n);
+    b->sopt.assign((size_t)n, {});
     b->device = ctxs[0]->device;
     {
         lvbgs_batch_options d;
@@ -910,8 +928,6 @@ extern "C" int lvbgs_batch_set_options(lvbgs_batch* b, const lvbgs_batch_options
         o->fixpoint_launches < 0 || o->fixpoint_launches > 64 || o->fixpoint_blocks < 1 || o->fixpoint_blocks > 65535 ||
         o->cont_px_per_block < 0 || o->fix_px_per_block < 0 || (o->fetch_depth != 2 && o->fetch_depth != 4))
         return bset_err(b, LVBGS_ERR_ARG, "invalid batch options");
-    hipSetDevice(b->device);
-    hipStreamSynchronize(b->stream);
     b->use_graph = o->use_graph != 0;
     b->opt.seq_limit = o->seq_limit;
     b->opt.cont_blocks = o->cont_blocks;
@@ -920,13 +936,43 @@ extern "C" int lvbgs_batch_set_options(lvbgs_batch* b, const lvbgs_batch_options
     b->opt.fixpoint_blocks = o->fixpoint_blocks;
     b->opt.fix_px_per_block = o->fix_px_per_block;
     b->opt.fetch_depth = o->fetch_depth;
-    return batch_plan(b);
+    b->dirty = true;
+    return LVBGS_OK;
+}
+
+extern "C" int lvbgs_batch_set_stream_options(lvbgs_batch* b, int index, int seq_limit, int cont_px_per_block,
+                                              int fix_px_per_block) {
+    if (!b) return LVBGS_ERR_ARG;
+    if (index < 0 || index >= (int)b->ctx.size() || seq_limit < 0 || seq_limit > 255 || cont_px_per_block < 0 ||
+        fix_px_per_block < 0)
+        return bset_err(b, LVBGS_ERR_ARG, "invalid stream options");
+    b->sopt[(size_t)index] = {seq_limit, cont_px_per_block, fix_px_per_block};
+    b->dirty = true;
+    return LVBGS_OK;
+}
+
+extern "C" int lvbgs_batch_get_stream_options(const lvbgs_batch* b, int index, int* seq_limit, int* cont_px_per_block,
+                                              int* fix_px_per_block) {
+    if (!b || index < 0 || index >= (int)b->ctx.size() || !seq_limit || !cont_px_per_block || !fix_px_per_block)
+        return LVBGS_ERR_ARG;
+    const auto& so = b->sopt[(size_t)index];
+    *seq_limit = so.seq_limit;
+    *cont_px_per_block = so.cont_px_per_block;
+    *fix_px_per_block = so.fix_px_per_block;
+    return LVBGS_OK;
 }
 
 extern "C" int lvbgs_batch_apply_device(lvbgs_batch* b, const void* const* d_imgs, void* const* d_masks,
                                         const double* lrs) {
     if (!b || !d_imgs) return LVBGS_ERR_ARG;
     hipSetDevice(b->device);
+    if (b->dirty) {
+        // the grid tables are rewritten: steps still running read them
+        if (hipStreamSynchronize(b->stream) != hipSuccess) return bset_err(b, LVBGS_ERR_HIP, "sync");
+        const int rc = batch_plan(b);
+        if (rc) return rc;
+        b->dirty = false;
+    }
     const int ns = (int)b->ctx.size(), i = b->flip;
     b->flip ^= 1;
     // the staging buffer's previous copy (two steps ago) must have been consumed
@@ -937,7 +983,7 @@ extern "C" int lvbgs_batch_apply_device(lvbgs_batch* b, const void* const* d_img
         v.frame = (const uint8_t*)d_imgs[s];
         v.out_mask = d_masks ? (uint8_t*)d_masks[s] : nullptr;
         v.lr = lrs ? lrs[s] : 0.0;
-        v.seq_limit = (uint32_t)b->opt.seq_limit;
+        v.seq_limit = (uint32_t)stream_options(b, s).seq_limit;
         b->h_views[i][s] = v;
         ++b->ctx[s]->apply_count;
     }

@@ -42,8 +42,10 @@ def _upload(torch, frames):
     return [torch.from_numpy(np.ascontiguousarray(f)).cuda() for f in frames]
 
 
-@pytest.mark.parametrize("use_graph", [False, True])
-def test_batch_matches_oracle_per_stream(use_graph):
+@pytest.mark.parametrize("use_graph,vary", [(False, False), (True, False), (True, True)])
+def test_batch_matches_oracle_per_stream(use_graph, vary):
+    """vary: every frame gives every stream different launch-shape overrides (sequential scan depth,
+    pixels per block of the continuation and fix-point phases), as the autotuner's trials do."""
     torch = _torch()
     specs = _streams()
     gpus, orcs, dframes, dmasks = [], [], [], []
@@ -59,7 +61,12 @@ def test_batch_matches_oracle_per_stream(use_graph):
     torch.cuda.synchronize()
     batch = Batch(gpus, use_graph=use_graph)
     n_frames = len(specs[0][0])
+    seq_limits, cont_ppb, fix_ppb = [1, 3, 8, 0, 255], [256, 4096, 0], [512, 0, 65536]
     for t in range(n_frames):
+        if vary:
+            for i in range(len(specs)):
+                batch.set_stream_options(i, seq_limit=seq_limits[(t + i) % 5], cont_px_per_block=cont_ppb[(t + 2 * i) % 3],
+                                         fix_px_per_block=fix_ppb[(t + i) % 3])
         batch.apply_device([d[t].data_ptr() for d in dframes], [m.data_ptr() for m in dmasks],
                            [s[2] for s in specs])
         batch.synchronize()
@@ -116,3 +123,11 @@ def test_batch_rejects_bad_members():
         Batch([s, lo])  # LOBSTER is stepped alone
     with pytest.raises(LvbgsError):
         Batch([s, s])  # duplicate member
+    b = Batch([s])
+    with pytest.raises(LvbgsError):
+        b.set_stream_options(1, seq_limit=8)  # no such member
+    with pytest.raises(LvbgsError):
+        b.set_stream_options(0, seq_limit=256)
+    b.set_stream_options(0, seq_limit=8, cont_px_per_block=1024)
+    assert b.stream_options(0) == {"seq_limit": 8, "cont_px_per_block": 1024, "fix_px_per_block": 0}
+    b.close()
