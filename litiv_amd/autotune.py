@@ -180,8 +180,8 @@ def tune_batch(batch, frame_ptrs, n_steps, candidates=None, log=None):
     using every step as a trial of the shared candidates in turn; leaves the best options set and
     returns them."""
     cands = list(candidates or SHARED_CANDIDATES)
-    tuner = OnlineTuner(batch, cands, classes=[], period=len(cands), max_period=len(cands),
-                        window=max(1, n_steps // len(cands)), log=log)
+    tuner = OnlineTuner(batch, cands, classes=[], period=0, max_period=0, window=max(1, n_steps // len(cands)),
+                        log=log)
     rounds = n_steps // len(cands)
     for t in range(rounds * len(cands)):
         tuner.apply(*frame_ptrs(t))
