@@ -103,21 +103,21 @@ extern "C" int lvbgs_create(int algo, const lvbgs_params* params, lvbgs_ctx** ou
 }
 
 static void release_device(lvbgs_ctx* c) {
-    if (c->arena) hipFree(c->arena);
+    if (c->arena) (void)hipFree(c->arena);
     c->arena = nullptr;
     c->arena_size = 0;
 }
 
 extern "C" void lvbgs_destroy(lvbgs_ctx* c) {
     if (!c) return;
-    hipSetDevice(c->device);
-    if (c->stream) hipStreamSynchronize(c->stream);
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (auto& e : c->ev_pairs) {
-        hipEventDestroy(e.first);
-        hipEventDestroy(e.second);
+        (void)hipEventDestroy(e.first);
+        (void)hipEventDestroy(e.second);
     }
     release_device(c);
-    if (c->stream) hipStreamDestroy(c->stream);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
 
@@ -405,14 +405,16 @@ static int check_lr(lvbgs_ctx* c, double lr) {
     return LVBGS_OK;
 }
 
-static void step(lvbgs_ctx* c, const uint8_t* d_img, uint8_t* d_mask, double lr) {
+static int step(lvbgs_ctx* c, const uint8_t* d_img, uint8_t* d_mask, double lr) {
     StepOptions o = c->opt;
     o.timed_stage = 0;
     if (c->timed_stage) {
         if (c->ev_used == c->ev_pairs.size()) {
-            hipEvent_t a, b;
-            hipEventCreate(&a);
-            hipEventCreate(&b);
+            hipEvent_t a = nullptr, b = nullptr;
+            if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) {
+                if (a) (void)hipEventDestroy(a);
+                return set_err(c, LVBGS_ERR_HIP, "timing events");
+            }
             c->ev_pairs.emplace_back(a, b);
         }
         o.timed_stage = c->timed_stage;
@@ -427,6 +429,7 @@ static void step(lvbgs_ctx* c, const uint8_t* d_img, uint8_t* d_mask, double lr)
     v.seq_limit = (uint32_t)o.seq_limit;
     frame_step(v, o, c->stream);
     ++c->apply_count;
+    return LVBGS_OK;
 }
 
 extern "C" int lvbgs_apply(lvbgs_ctx* c, const uint8_t* img, size_t stride, uint8_t* fgmask, size_t mask_stride,
@@ -436,7 +439,7 @@ extern "C" int lvbgs_apply(lvbgs_ctx* c, const uint8_t* img, size_t stride, uint
     if (!img) return set_err(c, LVBGS_ERR_ARG, "input image type/size mismatch with initialization type/size");
     int rc;
     if ((rc = check_lr(c, lr))) return rc;
-    hipSetDevice(c->device);
+    (void)hipSetDevice(c->device);
     const size_t row = (size_t)c->W * c->C;
     if (stride == 0) stride = row;
     if (mask_stride == 0) mask_stride = (size_t)c->W;
@@ -444,7 +447,7 @@ extern "C" int lvbgs_apply(lvbgs_ctx* c, const uint8_t* img, size_t stride, uint
     if ((rc = hip_check(c, hipMemcpy2DAsync(c->d_frame, row, img, stride, row, (size_t)c->H, hipMemcpyHostToDevice,
                                             c->stream), "input copy")))
         return rc;
-    step(c, c->d_frame, c->d_mask, lr);
+    if ((rc = step(c, c->d_frame, c->d_mask, lr))) return rc;
     if ((rc = hip_check(c, hipGetLastError(), "apply launch"))) return rc;
     if (fgmask &&
         (rc = hip_check(c, hipMemcpy2DAsync(fgmask, mask_stride, c->d_mask, (size_t)c->W, (size_t)c->W, (size_t)c->H,
@@ -459,14 +462,14 @@ extern "C" int lvbgs_apply_device(lvbgs_ctx* c, const void* d_img, void* d_mask,
     if (!d_img) return set_err(c, LVBGS_ERR_ARG, "input image must be a device pointer");
     int rc;
     if ((rc = check_lr(c, lr))) return rc;
-    hipSetDevice(c->device);
-    step(c, (const uint8_t*)d_img, (uint8_t*)d_mask, lr);
+    (void)hipSetDevice(c->device);
+    if ((rc = step(c, (const uint8_t*)d_img, (uint8_t*)d_mask, lr))) return rc;
     return hip_check(c, hipGetLastError(), "apply launch");
 }
 
 extern "C" int lvbgs_synchronize(lvbgs_ctx* c) {
     if (!c) return LVBGS_ERR_ARG;
-    hipSetDevice(c->device);
+    (void)hipSetDevice(c->device);
     int rc = hip_check(c, hipStreamSynchronize(c->stream), "synchronize");
     if (rc || !c->initialized) return rc;
     uint32_t flags = 0;
@@ -479,7 +482,7 @@ extern "C" int lvbgs_synchronize(lvbgs_ctx* c) {
 extern "C" int lvbgs_get_background_image(lvbgs_ctx* c, uint8_t* out, size_t stride) {
     if (!c || !out) return LVBGS_ERR_ARG;
     if (!c->initialized) return set_err(c, LVBGS_ERR_STATE, "algo must be initialized first");
-    hipSetDevice(c->device);
+    (void)hipSetDevice(c->device);
     const size_t row = (size_t)c->W * c->C;
     if (stride == 0) stride = row;
     if (stride < row) return set_err(c, LVBGS_ERR_ARG, "stride too small");
@@ -493,7 +496,7 @@ extern "C" int lvbgs_get_background_image(lvbgs_ctx* c, uint8_t* out, size_t str
 extern "C" int lvbgs_get_background_descriptors_image(lvbgs_ctx* c, uint16_t* out, size_t stride) {
     if (!c || !out) return LVBGS_ERR_ARG;
     if (!c->initialized) return set_err(c, LVBGS_ERR_STATE, "algo must be initialized first");
-    hipSetDevice(c->device);
+    (void)hipSetDevice(c->device);
     const size_t row = (size_t)c->W * c->C * 2;
     if (stride == 0) stride = row;
     if (stride < row) return set_err(c, LVBGS_ERR_ARG, "stride too small");
@@ -509,7 +512,7 @@ extern "C" int lvbgs_get_background_descriptors_image(lvbgs_ctx* c, uint16_t* ou
 extern "C" int lvbgs_set_automatic_model_reset(lvbgs_ctx* c, int enabled) {
     if (!c) return LVBGS_ERR_ARG;
     if (!c->initialized) return LVBGS_OK;  // initialize() sets the flag itself (SUB:113-123)
-    hipSetDevice(c->device);
+    (void)hipSetDevice(c->device);
     const int32_t val = enabled ? 1 : 0;
     int rc = hip_check(c, hipMemcpyAsync(&c->v.sc->auto_reset, &val, sizeof(val), hipMemcpyHostToDevice, c->stream),
                        "auto reset");
@@ -521,7 +524,7 @@ extern "C" int lvbgs_refresh_model(lvbgs_ctx* c, float frac, int force_fg) {
     if (!c) return LVBGS_ERR_ARG;
     if (!c->initialized) return set_err(c, LVBGS_ERR_STATE, "algo must be initialized first");
     if (!(frac > 0.0f && frac <= 1.0f)) return set_err(c, LVBGS_ERR_ARG, "model refresh must be given as a non-null fraction");
-    hipSetDevice(c->device);
+    (void)hipSetDevice(c->device);
     Scalars s;
     int rc = download_scalars(c, s);
     if (rc) return rc;
@@ -613,7 +616,7 @@ extern "C" int lvbgs_get_scalars(lvbgs_ctx* c, double* out, int cap) {
 extern "C" long long lvbgs_get_state(lvbgs_ctx* c, const char* name, void* out, size_t cap) {
     if (!c || !name || !out) return LVBGS_ERR_ARG;
     if (!c->initialized) return LVBGS_ERR_STATE;
-    hipSetDevice(c->device);
+    (void)hipSetDevice(c->device);
     const View& v = c->v;
     const size_t npx = c->npx, C = (size_t)c->C, N = c->p.bg_samples;
     const std::string n(name);
@@ -729,13 +732,13 @@ extern "C" int lvbgs_set_match_schedule(lvbgs_ctx* c, int seq_limit, int cont_bl
 // RNG self-check for tests: draws of group 0 for pixels [0,n) of a (frame, phase, event) counter
 extern "C" int lvbgs_debug_rng(lvbgs_ctx* c, uint32_t frame, uint32_t phase, uint32_t ev, uint32_t n, uint32_t* host_out) {
     if (!c || !host_out || !c->initialized || n == 0) return LVBGS_ERR_ARG;
-    hipSetDevice(c->device);
+    (void)hipSetDevice(c->device);
     uint32_t* d = nullptr;
     if (hipMalloc(&d, (size_t)n * 16) != hipSuccess) return LVBGS_ERR_ALLOC;
     launch_rng_probe(c->v, frame, phase, ev, n, d, c->stream);
     int rc = hip_check(c, hipMemcpyAsync(host_out, d, (size_t)n * 16, hipMemcpyDeviceToHost, c->stream), "rng copy");
     if (!rc) rc = hip_check(c, hipStreamSynchronize(c->stream), "sync");
-    hipFree(d);
+    (void)hipFree(d);
     return rc;
 }
 
@@ -743,7 +746,7 @@ extern "C" int lvbgs_debug_postprocess(lvbgs_ctx* c, const uint8_t* host_raw, ui
     if (!c || !host_raw || !host_last_fg) return LVBGS_ERR_ARG;
     if (!c->initialized) return set_err(c, LVBGS_ERR_STATE, "algo & model must be initialized first");
     if (c->algo != LVBGS_SUBSENSE) return set_err(c, LVBGS_ERR_ARG, "post-processing check is for SuBSENSE");
-    hipSetDevice(c->device);
+    (void)hipSetDevice(c->device);
     int rc = hip_check(c, hipMemcpyAsync(c->v.raw, host_raw, c->npx, hipMemcpyHostToDevice, c->stream), "raw copy");
     if (rc) return rc;
     launch_postprocess_probe(c->v, c->stream);
@@ -775,7 +778,6 @@ struct lvbgs_batch {
     };
     std::vector<StreamOverride> sopt;
     bool dirty = false;  // options changed since the grids were planned: re-plan at the next step
-    bool any_cont = false;
     bool use_graph = false;
     hipGraphExec_t graph = nullptr;
     int timed = 0;
@@ -789,19 +791,19 @@ static int bset_err(lvbgs_batch* b, int code, const char* msg) {
 }
 
 static void batch_release(lvbgs_batch* b) {
-    if (b->graph) hipGraphExecDestroy(b->graph);
+    if (b->graph) (void)hipGraphExecDestroy(b->graph);
     b->graph = nullptr;
-    if (b->d_views) hipFree(b->d_views);
-    if (b->d_prefix) hipFree(b->d_prefix);
+    if (b->d_views) (void)hipFree(b->d_views);
+    if (b->d_prefix) (void)hipFree(b->d_prefix);
     for (int i = 0; i < 2; ++i) {
-        if (b->h_views[i]) hipHostFree(b->h_views[i]);
-        if (b->ev_copy[i]) hipEventDestroy(b->ev_copy[i]);
+        if (b->h_views[i]) (void)hipHostFree(b->h_views[i]);
+        if (b->ev_copy[i]) (void)hipEventDestroy(b->ev_copy[i]);
     }
     for (auto& e : b->ev_pairs) {
-        hipEventDestroy(e.first);
-        hipEventDestroy(e.second);
+        (void)hipEventDestroy(e.first);
+        (void)hipEventDestroy(e.second);
     }
-    if (b->stream) hipStreamDestroy(b->stream);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
 }
 
 // the options of stream i: the batch's, with the stream's overrides
@@ -817,13 +819,11 @@ static StepOptions stream_options(const lvbgs_batch* b, int i) {
 static int batch_plan(lvbgs_batch* b) {
     const int ns = (int)b->ctx.size();
     std::vector<uint32_t> pre((size_t)kNumGrids * (ns + 1), 0);
-    b->any_cont = false;
     for (int i = 0; i < ns; ++i) {
         uint32_t g[kNumGrids];
         const StepOptions o = stream_options(b, i);
         kernel_grids(b->ctx[i]->v, o, g);
         for (int k = 0; k < kNumGrids; ++k) pre[(size_t)k * (ns + 1) + i + 1] = pre[(size_t)k * (ns + 1) + i] + g[k];
-        if (o.seq_limit < b->ctx[i]->v.N) b->any_cont = true;
     }
     if (!b->d_prefix && hipMalloc(&b->d_prefix, pre.size() * sizeof(uint32_t)) != hipSuccess)
         return bset_err(b, LVBGS_ERR_ALLOC, "batch tables");
@@ -833,7 +833,7 @@ static int batch_plan(lvbgs_batch* b) {
         b->prefix[k] = b->d_prefix + (size_t)k * (ns + 1);
         b->total[k] = pre[(size_t)k * (ns + 1) + ns];
     }
-    if (b->graph) hipGraphExecDestroy(b->graph);
+    if (b->graph) (void)hipGraphExecDestroy(b->graph);
     b->graph = nullptr;
     return LVBGS_OK;
 }
@@ -863,7 +863,7 @@ extern "C" int lvbgs_batch_create(lvbgs_ctx* const* ctxs, int n, lvbgs_batch** o
         b->opt.fix_px_per_block = d.fix_px_per_block;
         b->opt.fetch_depth = d.fetch_depth;
     }
-    hipSetDevice(b->device);
+    (void)hipSetDevice(b->device);
     int rc = LVBGS_OK;
     if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
         hipMalloc(&b->d_views, sizeof(View) * n) != hipSuccess ||
@@ -879,15 +879,15 @@ extern "C" int lvbgs_batch_create(lvbgs_ctx* const* ctxs, int n, lvbgs_batch** o
         return rc;
     }
     // member streams must be idle before the batch takes over their state
-    for (auto* c : b->ctx) hipStreamSynchronize(c->stream);
+    for (auto* c : b->ctx) (void)hipStreamSynchronize(c->stream);
     *out = b;
     return LVBGS_OK;
 }
 
 extern "C" void lvbgs_batch_destroy(lvbgs_batch* b) {
     if (!b) return;
-    hipSetDevice(b->device);
-    if (b->stream) hipStreamSynchronize(b->stream);
+    (void)hipSetDevice(b->device);
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
     batch_release(b);
     delete b;
 }
@@ -965,7 +965,7 @@ extern "C" int lvbgs_batch_get_stream_options(const lvbgs_batch* b, int index, i
 extern "C" int lvbgs_batch_apply_device(lvbgs_batch* b, const void* const* d_imgs, void* const* d_masks,
                                         const double* lrs) {
     if (!b || !d_imgs) return LVBGS_ERR_ARG;
-    hipSetDevice(b->device);
+    (void)hipSetDevice(b->device);
     if (b->dirty) {
         // the grid tables are rewritten: steps still running read them
         if (hipStreamSynchronize(b->stream) != hipSuccess) return bset_err(b, LVBGS_ERR_HIP, "sync");
@@ -995,9 +995,11 @@ extern "C" int lvbgs_batch_apply_device(lvbgs_batch* b, const void* const* d_img
     o.timed_stage = 0;
     if (b->timed) {
         if (b->ev_used == b->ev_pairs.size()) {
-            hipEvent_t e0, e1;
-            hipEventCreate(&e0);
-            hipEventCreate(&e1);
+            hipEvent_t e0 = nullptr, e1 = nullptr;
+            if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+                if (e0) (void)hipEventDestroy(e0);
+                return bset_err(b, LVBGS_ERR_HIP, "timing events");
+            }
             b->ev_pairs.emplace_back(e0, e1);
         }
         o.ev_begin = b->ev_pairs[b->ev_used].first;
@@ -1007,18 +1009,18 @@ extern "C" int lvbgs_batch_apply_device(lvbgs_batch* b, const void* const* d_img
     if (b->timed == 1) {
         // kernel-stage timing: events between the launches, so the step is issued directly
         o.timed_stage = 1;
-        batch_step(b->d_views, ns, b->prefix, b->total, o, b->any_cont, b->stream);
+        batch_step(b->d_views, ns, b->prefix, b->total, o, b->stream);
     } else if (b->use_graph) {
         if (b->timed == 2) (void)hipEventRecord(o.ev_begin, b->stream);
         if (!b->graph) {
             hipGraph_t g = nullptr;
             if (hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
                 return bset_err(b, LVBGS_ERR_HIP, "graph capture");
-            batch_step(b->d_views, ns, b->prefix, b->total, b->opt, b->any_cont, b->stream);
+            batch_step(b->d_views, ns, b->prefix, b->total, b->opt, b->stream);
             if (hipStreamEndCapture(b->stream, &g) != hipSuccess || !g)
                 return bset_err(b, LVBGS_ERR_HIP, "graph capture");
             const hipError_t e = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
-            hipGraphDestroy(g);
+            (void)hipGraphDestroy(g);
             if (e != hipSuccess) {
                 b->graph = nullptr;
                 return bset_err(b, LVBGS_ERR_HIP, "graph instantiate");
@@ -1028,14 +1030,14 @@ extern "C" int lvbgs_batch_apply_device(lvbgs_batch* b, const void* const* d_img
         if (b->timed == 2) (void)hipEventRecord(o.ev_end, b->stream);
     } else {
         o.timed_stage = b->timed;
-        batch_step(b->d_views, ns, b->prefix, b->total, o, b->any_cont, b->stream);
+        batch_step(b->d_views, ns, b->prefix, b->total, o, b->stream);
     }
     return hipGetLastError() == hipSuccess ? LVBGS_OK : bset_err(b, LVBGS_ERR_HIP, "batch launch");
 }
 
 extern "C" int lvbgs_batch_synchronize(lvbgs_batch* b) {
     if (!b) return LVBGS_ERR_ARG;
-    hipSetDevice(b->device);
+    (void)hipSetDevice(b->device);
     if (hipStreamSynchronize(b->stream) != hipSuccess) return bset_err(b, LVBGS_ERR_HIP, "synchronize");
     for (auto* c : b->ctx) {
         uint32_t flags = 0;
@@ -1053,8 +1055,8 @@ extern "C" int lvbgs_batch_set_timing(lvbgs_batch* b, const char* stage) {
     else if (s == "match") b->timed = 1;
     else if (s == "step") b->timed = 2;
     else return bset_err(b, LVBGS_ERR_ARG, "timing stage must be \"match\", \"step\" or \"\"");
-    hipSetDevice(b->device);
-    hipStreamSynchronize(b->stream);
+    (void)hipSetDevice(b->device);
+    (void)hipStreamSynchronize(b->stream);
     b->ev_used = 0;
     return LVBGS_OK;
 }

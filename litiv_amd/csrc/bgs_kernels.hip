@@ -623,7 +623,7 @@ LV_D void describe_body(const View& v, uint32_t bid, uint32_t nblk) {
     const double lr = v.lr;
     __shared__ FrameTile<C> tile;
     __shared__ uint8_t lut[256];
-    const int ntx = (v.W + TW - 1) / TW, nty = (v.H + TH - 1) / TH;
+    const int ntx = (v.W + TW - 1) / TW;
     const int tid = (int)bid;
     const int x0 = (tid % ntx) * TW, y0 = (tid / ntx) * TH;
     if (bid == 0 && threadIdx.x == 0) {
@@ -877,7 +877,7 @@ LV_D void match_body(const View& v, uint32_t bid, uint32_t nblk) {
     __shared__ FrameTile<C> tile;
     __shared__ PredTile pred;
     __shared__ uint8_t lut[256];
-    const int ntx = (v.W + TW - 1) / TW, nty = (v.H + TH - 1) / TH;
+    const int ntx = (v.W + TW - 1) / TW;
     const int tid = (int)bid;
     const int x0 = (tid % ntx) * TW, y0 = (tid / ntx) * TH;
     __shared__ uint16_t s_perm[TPB];
@@ -1308,7 +1308,7 @@ LV_D void update_body(const View& v, uint32_t bid, uint32_t nblk) {
     __shared__ uint8_t s_bg[LH][LW];
     __shared__ uint8_t s_col[LH][LW * C];
     __shared__ uint16_t s_desc[LH][LW * C];
-    const int ntx = (v.W + TW - 1) / TW, nty = (v.H + TH - 1) / TH;
+    const int ntx = (v.W + TW - 1) / TW;
     const int tid = (int)bid;
     const int x0 = (tid % ntx) * TW, y0 = (tid / ntx) * TH;
     // neighbourhood loads first, into registers: their latency overlaps the feedback pass below
@@ -2406,7 +2406,7 @@ void frame_step(const View& v, const StepOptions& opt, hipStream_t st) {
 // one SuBSENSE step of every stream of a batch; d_views holds the streams' Views (with their
 // per-frame inputs), prefix[k] the (ns+1) block offsets of grid kind k, total[k] their sums
 void batch_step(const View* d_views, int ns, const uint32_t* const prefix[kNumGrids], const uint32_t total[kNumGrids],
-                const StepOptions& opt, bool any_cont, hipStream_t st) {
+                const StepOptions& opt, hipStream_t st) {
     auto args = [&](int kind) { return BatchArgs{d_views, prefix[kind], ns}; };
     if (opt.timed_stage == 2) (void)hipEventRecord(opt.ev_begin, st);
     hipLaunchKernelGGL(kb_describe, dim3(total[kGridTile]), dim3(TPB), 0, st, args(kGridTile));

@@ -43,7 +43,7 @@ void frame_step(const View& v, const StepOptions& opt, hipStream_t st);
 
 // one SuBSENSE step of every stream of a batch (see bgs_kernels.hip)
 void batch_step(const View* d_views, int ns, const uint32_t* const prefix[kNumGrids], const uint32_t total[kNumGrids],
-                const StepOptions& opt, bool any_cont, hipStream_t st);
+                const StepOptions& opt, hipStream_t st);
 void launch_refresh(const View& v, float frac, int force_fg, uint32_t frame, uint32_t ev, hipStream_t st);
 void launch_init_desc(const View& v, const uint8_t* img, hipStream_t st);
 void launch_background(const View& v, uint8_t* out_img, uint16_t* out_desc, hipStream_t st);

@@ -250,7 +250,7 @@ static int launch_step(lvbgs_server* s) {
         prefix[k] = s->d_prefix + (size_t)k * (n + 1);
         total[k] = hp[(size_t)k * (n + 1) + n];
     }
-    batch_step(s->d_views, n, prefix, total, s->opt, true, s->stream);
+    batch_step(s->d_views, n, prefix, total, s->opt, s->stream);
     q.done = s->ev_free.back();
     s->ev_free.pop_back();
     if (hipEventRecord(q.done, s->stream) != hipSuccess || hipGetLastError() != hipSuccess) {
@@ -265,7 +265,7 @@ static int launch_step(lvbgs_server* s) {
 
 extern "C" int lvbgs_server_pump(lvbgs_server* s) {
     if (!s) return LVBGS_ERR_ARG;
-    hipSetDevice(s->device);
+    (void)hipSetDevice(s->device);
     int rc = harvest(s, false);
     if (rc) return rc;
     int launched = 0;
@@ -290,7 +290,7 @@ extern "C" int lvbgs_server_poll(lvbgs_server* s, lvbgs_done* out, int cap) {
 
 extern "C" int lvbgs_server_drain(lvbgs_server* s) {
     if (!s) return LVBGS_ERR_ARG;
-    hipSetDevice(s->device);
+    (void)hipSetDevice(s->device);
     while (true) {
         int rc = lvbgs_server_pump(s);
         if (rc < 0) return rc;
@@ -309,7 +309,7 @@ extern "C" int lvbgs_server_open(lvbgs_server* s, const lvbgs_params* params, co
                                  int height, int channels, int* stream_out) {
     if (!s || !params || !d_img0 || !stream_out || width <= 0 || height <= 0 || (channels != 1 && channels != 3))
         return s ? srv_err(s, LVBGS_ERR_ARG, "invalid stream parameters") : LVBGS_ERR_ARG;
-    hipSetDevice(s->device);
+    (void)hipSetDevice(s->device);
     lvbgs_params p = *params;
     p.device = s->device;
     const size_t need = lvbgs_arena_bytes(LVBGS_SUBSENSE, p, width, height, channels);
@@ -411,7 +411,7 @@ extern "C" int lvbgs_server_get_stats(lvbgs_server* s, lvbgs_server_stats* out) 
 
 extern "C" void lvbgs_server_destroy(lvbgs_server* s) {
     if (!s) return;
-    hipSetDevice(s->device);
+    (void)hipSetDevice(s->device);
     if (s->stream) (void)hipStreamSynchronize(s->stream);
     server_release(s);
     delete s;
