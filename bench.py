@@ -250,6 +250,7 @@ def run_batch(args, preset, n_streams, ref_frames):
     batch = Batch(subs, **tuned)
     if tuner is not None:
         tuner.apply_choices(batch)
+    batch.set_options(groups=1)  # the roofline kernel: one match launch over every stream
     for t in range(args.warmup):
         batch.apply_device(*ptrs(t))
     batch.set_timing("match")

@@ -184,6 +184,7 @@ typedef struct lvbgs_batch_options {
     int fixpoint_blocks;    /* fix-point grid of a stream, at most */
     int fix_px_per_block;   /*   ... and ceil(pixels / this) when > 0 */
     int fetch_depth;        /* sample loads in flight per pixel in the sequential match (2 or 4) */
+    int groups;             /* 1..4: members split into groups stepped concurrently on their own streams */
 } lvbgs_batch_options;
 int lvbgs_batch_default_options(lvbgs_batch_options* out);
 int lvbgs_batch_get_options(const lvbgs_batch* b, lvbgs_batch_options* out);

@@ -4,7 +4,8 @@ The batch's launch options change the order of evaluation but never the results,
 live frames. Two kinds of option are tuned:
 
 * shared by every stream (one kernel variant per launch): sample loads in flight per lane of the
-  sequential match (`fetch_depth`), full-grid fix-point launches (`fixpoint_launches`);
+  sequential match (`fetch_depth`), full-grid fix-point launches (`fixpoint_launches`), number of
+  member groups stepped concurrently on their own streams (`groups`);
 * per resolution class (member overrides, `Batch.set_stream_options`): samples scanned by the
   sequential match phase before the dense continuation (`seq_limit`) and pixels per block of the
   continuation phase's grid-stride loop (`cont_px_per_block`) -- the launch shape each resolution
@@ -25,9 +26,11 @@ frames favour a full sequential scan, settled models a short one).
 import numpy as np
 
 SHARED_CANDIDATES = [
-    {"fixpoint_launches": 2, "fetch_depth": 4},
-    {"fixpoint_launches": 3, "fetch_depth": 4},
-    {"fixpoint_launches": 2, "fetch_depth": 2},
+    {"fixpoint_launches": 2, "fetch_depth": 4, "groups": 2},
+    {"fixpoint_launches": 2, "fetch_depth": 4, "groups": 1},
+    {"fixpoint_launches": 2, "fetch_depth": 4, "groups": 4},
+    {"fixpoint_launches": 3, "fetch_depth": 4, "groups": 2},
+    {"fixpoint_launches": 2, "fetch_depth": 2, "groups": 2},
 ]
 CLASS_CHOICES = {"seq_limit": [4, 6, 8, 12, 16, 24, 255], "cont_px_per_block": [512, 1024, 2048, 4096, 8192]}
 CLASS_START = {"seq_limit": 12, "cont_px_per_block": 2048}

@@ -59,7 +59,7 @@ class Stats(C.Structure):
 class BatchOptions(C.Structure):
     _fields_ = [("use_graph", C.c_int), ("seq_limit", C.c_int), ("cont_blocks", C.c_int),
                 ("cont_px_per_block", C.c_int), ("fixpoint_launches", C.c_int), ("fixpoint_blocks", C.c_int),
-                ("fix_px_per_block", C.c_int), ("fetch_depth", C.c_int)]
+                ("fix_px_per_block", C.c_int), ("fetch_depth", C.c_int), ("groups", C.c_int)]
 
 
 class ServerOptions(C.Structure):

@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <algorithm>
 #include <string>
 #include <vector>
 
@@ -769,9 +770,23 @@ struct lvbgs_batch {
     bool ev_live[2] = {false, false};
     int flip = 0;
     uint32_t* d_prefix = nullptr;
-    const uint32_t* prefix[kNumGrids] = {};
-    uint32_t total[kNumGrids] = {};
     StepOptions opt{};
+    // Members are split into `ngroups` groups (largest first, dealt round-robin) stepped on their
+    // own HIP streams between a fork and a join on the batch stream, so that one group's memory-bound
+    // kernels can overlap another's VALU-bound ones. order[k] = member staged at View slot k; a group
+    // is a contiguous range of slots.
+    static constexpr int kMaxGroups = 4;
+    struct Group {
+        int first = 0, count = 0;
+        const uint32_t* prefix[kNumGrids] = {};
+        uint32_t total[kNumGrids] = {};
+        hipGraphExec_t graph = nullptr;
+    };
+    int ngroups = 1, planned_groups = 1;
+    std::vector<int> order;
+    Group grp[kMaxGroups];
+    hipStream_t gstream[kMaxGroups] = {};  // gstream[0] is the batch stream
+    hipEvent_t ev_fork = nullptr, ev_join[kMaxGroups] = {};
     // per-stream overrides of the batch options (0 = the batch's value)
     struct StreamOverride {
         int seq_limit = 0, cont_px_per_block = 0, fix_px_per_block = 0;
@@ -779,7 +794,6 @@ struct lvbgs_batch {
     std::vector<StreamOverride> sopt;
     bool dirty = false;  // options changed since the grids were planned: re-plan at the next step
     bool use_graph = false;
-    hipGraphExec_t graph = nullptr;
     int timed = 0;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pairs;
     size_t ev_used = 0;
@@ -791,8 +805,15 @@ static int bset_err(lvbgs_batch* b, int code, const char* msg) {
 }
 
 static void batch_release(lvbgs_batch* b) {
-    if (b->graph) (void)hipGraphExecDestroy(b->graph);
-    b->graph = nullptr;
+    for (auto& g : b->grp) {
+        if (g.graph) (void)hipGraphExecDestroy(g.graph);
+        g.graph = nullptr;
+    }
+    for (int g = 1; g < lvbgs_batch::kMaxGroups; ++g)
+        if (b->gstream[g]) (void)hipStreamDestroy(b->gstream[g]);
+    for (auto e : b->ev_join)
+        if (e) (void)hipEventDestroy(e);
+    if (b->ev_fork) (void)hipEventDestroy(b->ev_fork);
     if (b->d_views) (void)hipFree(b->d_views);
     if (b->d_prefix) (void)hipFree(b->d_prefix);
     for (int i = 0; i < 2; ++i) {
@@ -818,23 +839,57 @@ static StepOptions stream_options(const lvbgs_batch* b, int i) {
 
 static int batch_plan(lvbgs_batch* b) {
     const int ns = (int)b->ctx.size();
-    std::vector<uint32_t> pre((size_t)kNumGrids * (ns + 1), 0);
-    for (int i = 0; i < ns; ++i) {
-        uint32_t g[kNumGrids];
-        const StepOptions o = stream_options(b, i);
-        kernel_grids(b->ctx[i]->v, o, g);
-        for (int k = 0; k < kNumGrids; ++k) pre[(size_t)k * (ns + 1) + i + 1] = pre[(size_t)k * (ns + 1) + i] + g[k];
+    const int G = std::max(1, std::min(b->ngroups, ns));
+    // largest members first, dealt round-robin: every group gets a share of each resolution
+    std::vector<int> by_size(ns);
+    for (int i = 0; i < ns; ++i) by_size[i] = i;
+    std::stable_sort(by_size.begin(), by_size.end(),
+                     [&](int x, int y) { return b->ctx[x]->npx > b->ctx[y]->npx; });
+    b->order.clear();
+    for (int g = 0; g < lvbgs_batch::kMaxGroups; ++g) {
+        auto& gr = b->grp[g];
+        if (gr.graph) (void)hipGraphExecDestroy(gr.graph);
+        gr.graph = nullptr;
+        gr.first = (int)b->order.size();
+        if (g < G)
+            for (int k = g; k < ns; k += G) b->order.push_back(by_size[k]);
+        gr.count = (int)b->order.size() - gr.first;
+    }
+    // grid tables: group g, kind k at row (g * kNumGrids + k), ns + 1 entries per row
+    const size_t row = (size_t)ns + 1;
+    std::vector<uint32_t> pre((size_t)lvbgs_batch::kMaxGroups * kNumGrids * row, 0);
+    for (int g = 0; g < G; ++g) {
+        const auto& gr = b->grp[g];
+        for (int j = 0; j < gr.count; ++j) {
+            const int i = b->order[(size_t)gr.first + j];
+            uint32_t gk[kNumGrids];
+            kernel_grids(b->ctx[i]->v, stream_options(b, i), gk);
+            for (int k = 0; k < kNumGrids; ++k) {
+                uint32_t* r = &pre[((size_t)g * kNumGrids + k) * row];
+                r[j + 1] = r[j] + gk[k];
+            }
+        }
     }
     if (!b->d_prefix && hipMalloc(&b->d_prefix, pre.size() * sizeof(uint32_t)) != hipSuccess)
         return bset_err(b, LVBGS_ERR_ALLOC, "batch tables");
     if (hipMemcpy(b->d_prefix, pre.data(), pre.size() * sizeof(uint32_t), hipMemcpyHostToDevice) != hipSuccess)
         return bset_err(b, LVBGS_ERR_HIP, "batch tables");
-    for (int k = 0; k < kNumGrids; ++k) {
-        b->prefix[k] = b->d_prefix + (size_t)k * (ns + 1);
-        b->total[k] = pre[(size_t)k * (ns + 1) + ns];
+    for (int g = 0; g < G; ++g)
+        for (int k = 0; k < kNumGrids; ++k) {
+            b->grp[g].prefix[k] = b->d_prefix + ((size_t)g * kNumGrids + k) * row;
+            b->grp[g].total[k] = pre[((size_t)g * kNumGrids + k) * row + b->grp[g].count];
+        }
+    // group streams and fork/join events, created on first use
+    for (int g = 1; g < G; ++g) {
+        if (!b->gstream[g] && hipStreamCreateWithFlags(&b->gstream[g], hipStreamNonBlocking) != hipSuccess)
+            return bset_err(b, LVBGS_ERR_HIP, "group stream");
+        if (!b->ev_join[g] && hipEventCreateWithFlags(&b->ev_join[g], hipEventDisableTiming) != hipSuccess)
+            return bset_err(b, LVBGS_ERR_HIP, "group event");
     }
-    if (b->graph) (void)hipGraphExecDestroy(b->graph);
-    b->graph = nullptr;
+    if (G > 1 && !b->ev_fork && hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming) != hipSuccess)
+        return bset_err(b, LVBGS_ERR_HIP, "group event");
+    b->gstream[0] = b->stream;
+    b->planned_groups = G;
     return LVBGS_OK;
 }
 
@@ -862,6 +917,7 @@ extern "C" int lvbgs_batch_create(lvbgs_ctx* const* ctxs, int n, lvbgs_batch** o
         b->opt.fixpoint_blocks = d.fixpoint_blocks;
         b->opt.fix_px_per_block = d.fix_px_per_block;
         b->opt.fetch_depth = d.fetch_depth;
+        b->ngroups = d.groups;
     }
     (void)hipSetDevice(b->device);
     int rc = LVBGS_OK;
@@ -906,6 +962,7 @@ extern "C" int lvbgs_batch_default_options(lvbgs_batch_options* o) {
     o->fixpoint_blocks = 256;
     o->fix_px_per_block = 8192;
     o->fetch_depth = 4;
+    o->groups = 1;
     return LVBGS_OK;
 }
 
@@ -919,6 +976,7 @@ extern "C" int lvbgs_batch_get_options(const lvbgs_batch* b, lvbgs_batch_options
     o->fixpoint_blocks = b->opt.fixpoint_blocks;
     o->fix_px_per_block = b->opt.fix_px_per_block;
     o->fetch_depth = b->opt.fetch_depth;
+    o->groups = b->ngroups;
     return LVBGS_OK;
 }
 
@@ -926,7 +984,8 @@ extern "C" int lvbgs_batch_set_options(lvbgs_batch* b, const lvbgs_batch_options
     if (!b || !o) return LVBGS_ERR_ARG;
     if (o->seq_limit < 1 || o->seq_limit > 255 || o->cont_blocks < 1 || o->cont_blocks > 65535 ||
         o->fixpoint_launches < 0 || o->fixpoint_launches > 64 || o->fixpoint_blocks < 1 || o->fixpoint_blocks > 65535 ||
-        o->cont_px_per_block < 0 || o->fix_px_per_block < 0 || (o->fetch_depth != 2 && o->fetch_depth != 4))
+        o->cont_px_per_block < 0 || o->fix_px_per_block < 0 || (o->fetch_depth != 2 && o->fetch_depth != 4) ||
+        o->groups < 1 || o->groups > lvbgs_batch::kMaxGroups)
         return bset_err(b, LVBGS_ERR_ARG, "invalid batch options");
     b->use_graph = o->use_graph != 0;
     b->opt.seq_limit = o->seq_limit;
@@ -936,6 +995,7 @@ extern "C" int lvbgs_batch_set_options(lvbgs_batch* b, const lvbgs_batch_options
     b->opt.fixpoint_blocks = o->fixpoint_blocks;
     b->opt.fix_px_per_block = o->fix_px_per_block;
     b->opt.fetch_depth = o->fetch_depth;
+    b->ngroups = o->groups;
     b->dirty = true;
     return LVBGS_OK;
 }
@@ -977,14 +1037,18 @@ extern "C" int lvbgs_batch_apply_device(lvbgs_batch* b, const void* const* d_img
     b->flip ^= 1;
     // the staging buffer's previous copy (two steps ago) must have been consumed
     if (b->ev_live[i] && hipEventSynchronize(b->ev_copy[i]) != hipSuccess) return bset_err(b, LVBGS_ERR_HIP, "sync");
-    for (int s = 0; s < ns; ++s) {
+    for (int s = 0; s < ns; ++s)
         if (!d_imgs[s]) return bset_err(b, LVBGS_ERR_ARG, "input image must be a device pointer");
+    if (b->timed == 1 && b->planned_groups > 1)
+        return bset_err(b, LVBGS_ERR_ARG, "match-stage timing needs groups = 1");
+    for (int k = 0; k < ns; ++k) {
+        const int s = b->order[(size_t)k];
         View v = b->ctx[s]->v;
         v.frame = (const uint8_t*)d_imgs[s];
         v.out_mask = d_masks ? (uint8_t*)d_masks[s] : nullptr;
         v.lr = lrs ? lrs[s] : 0.0;
         v.seq_limit = (uint32_t)stream_options(b, s).seq_limit;
-        b->h_views[i][s] = v;
+        b->h_views[i][k] = v;
         ++b->ctx[s]->apply_count;
     }
     if (hipMemcpyAsync(b->d_views, b->h_views[i], sizeof(View) * ns, hipMemcpyHostToDevice, b->stream) != hipSuccess ||
@@ -1006,32 +1070,51 @@ extern "C" int lvbgs_batch_apply_device(lvbgs_batch* b, const void* const* d_img
         o.ev_end = b->ev_pairs[b->ev_used].second;
         ++b->ev_used;
     }
+    const int G = b->planned_groups;
     if (b->timed == 1) {
-        // kernel-stage timing: events between the launches, so the step is issued directly
+        // kernel-stage timing (one group): events between the launches, so the step is issued directly
         o.timed_stage = 1;
-        batch_step(b->d_views, ns, b->prefix, b->total, o, b->stream);
-    } else if (b->use_graph) {
-        if (b->timed == 2) (void)hipEventRecord(o.ev_begin, b->stream);
-        if (!b->graph) {
-            hipGraph_t g = nullptr;
-            if (hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
+        batch_step(b->d_views, ns, b->grp[0].prefix, b->grp[0].total, o, b->stream);
+        return hipGetLastError() == hipSuccess ? LVBGS_OK : bset_err(b, LVBGS_ERR_HIP, "batch launch");
+    }
+    if (b->timed == 2) (void)hipEventRecord(o.ev_begin, b->stream);
+    if (G > 1) {
+        // fork: the group streams start after the View upload
+        if (hipEventRecord(b->ev_fork, b->stream) != hipSuccess) return bset_err(b, LVBGS_ERR_HIP, "fork");
+        for (int g = 1; g < G; ++g)
+            if (hipStreamWaitEvent(b->gstream[g], b->ev_fork, 0) != hipSuccess) return bset_err(b, LVBGS_ERR_HIP, "fork");
+    }
+    StepOptions og = b->opt;
+    og.timed_stage = 0;
+    for (int g = 0; g < G; ++g) {
+        auto& gr = b->grp[g];
+        hipStream_t st = b->gstream[g];
+        const View* dv = b->d_views + gr.first;
+        if (!b->use_graph) {
+            batch_step(dv, gr.count, gr.prefix, gr.total, og, st);
+            continue;
+        }
+        if (!gr.graph) {
+            hipGraph_t gh = nullptr;
+            if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess)
                 return bset_err(b, LVBGS_ERR_HIP, "graph capture");
-            batch_step(b->d_views, ns, b->prefix, b->total, b->opt, b->stream);
-            if (hipStreamEndCapture(b->stream, &g) != hipSuccess || !g)
-                return bset_err(b, LVBGS_ERR_HIP, "graph capture");
-            const hipError_t e = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
-            (void)hipGraphDestroy(g);
+            batch_step(dv, gr.count, gr.prefix, gr.total, og, st);
+            if (hipStreamEndCapture(st, &gh) != hipSuccess || !gh) return bset_err(b, LVBGS_ERR_HIP, "graph capture");
+            const hipError_t e = hipGraphInstantiate(&gr.graph, gh, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(gh);
             if (e != hipSuccess) {
-                b->graph = nullptr;
+                gr.graph = nullptr;
                 return bset_err(b, LVBGS_ERR_HIP, "graph instantiate");
             }
         }
-        if (hipGraphLaunch(b->graph, b->stream) != hipSuccess) return bset_err(b, LVBGS_ERR_HIP, "graph launch");
-        if (b->timed == 2) (void)hipEventRecord(o.ev_end, b->stream);
-    } else {
-        o.timed_stage = b->timed;
-        batch_step(b->d_views, ns, b->prefix, b->total, o, b->stream);
+        if (hipGraphLaunch(gr.graph, st) != hipSuccess) return bset_err(b, LVBGS_ERR_HIP, "graph launch");
     }
+    // join: the batch stream waits for every group
+    for (int g = 1; g < G; ++g)
+        if (hipEventRecord(b->ev_join[g], b->gstream[g]) != hipSuccess ||
+            hipStreamWaitEvent(b->stream, b->ev_join[g], 0) != hipSuccess)
+            return bset_err(b, LVBGS_ERR_HIP, "join");
+    if (b->timed == 2) (void)hipEventRecord(o.ev_end, b->stream);
     return hipGetLastError() == hipSuccess ? LVBGS_OK : bset_err(b, LVBGS_ERR_HIP, "batch launch");
 }
 

@@ -42,10 +42,12 @@ def _upload(torch, frames):
     return [torch.from_numpy(np.ascontiguousarray(f)).cuda() for f in frames]
 
 
-@pytest.mark.parametrize("use_graph,vary", [(False, False), (True, False), (True, True)])
-def test_batch_matches_oracle_per_stream(use_graph, vary):
+@pytest.mark.parametrize("use_graph,vary,groups", [(False, False, 1), (True, False, 1), (True, True, 2),
+                                                   (True, False, 3), (False, True, 4)])
+def test_batch_matches_oracle_per_stream(use_graph, vary, groups):
     """vary: every frame gives every stream different launch-shape overrides (sequential scan depth,
-    pixels per block of the continuation and fix-point phases), as the autotuner's trials do."""
+    pixels per block of the continuation and fix-point phases), as the autotuner's trials do.
+    groups: members stepped in that many concurrent groups."""
     torch = _torch()
     specs = _streams()
     gpus, orcs, dframes, dmasks = [], [], [], []
@@ -59,7 +61,7 @@ def test_batch_matches_oracle_per_stream(use_graph, vary):
         h, w = frames[0].shape[:2]
         dmasks.append(torch.empty((h, w), dtype=torch.uint8, device="cuda"))
     torch.cuda.synchronize()
-    batch = Batch(gpus, use_graph=use_graph)
+    batch = Batch(gpus, use_graph=use_graph, groups=groups)
     n_frames = len(specs[0][0])
     seq_limits, cont_ppb, fix_ppb = [1, 3, 8, 0, 255], [256, 4096, 0], [512, 0, 65536]
     for t in range(n_frames):
