@@ -364,7 +364,7 @@ def run_serve(args, n_slots, log_prefix="serve"):
     fps, mean_life, ring = 30.0, 2000.0, 16
     rng = np.random.default_rng(0x5E12E)
     srv = Server(hbm_budget=250 << 30, max_inflight=2, max_batch=n_slots, fix_px_per_block=8192,
-                 cont_px_per_block=2048, seq_limit=8)
+                 cont_px_per_block=2048, seq_limit=8, groups=2)
     dev = torch.device("cuda:0")
     incarnation = [0]
     slots = {}      # server stream id -> dict

@@ -68,6 +68,11 @@ struct lvbgs_server {
     std::vector<lvbgs_done> done;
     int rr = 0;
     lvbgs_server_stats st{};
+    // a step's frames in `groups` groups stepped concurrently (fork/join on `stream`)
+    static constexpr int kMaxGroups = 4;
+    int groups = 1;
+    hipStream_t gstream[kMaxGroups] = {};  // gstream[0] is `stream`
+    hipEvent_t ev_fork = nullptr, ev_join[kMaxGroups] = {};
 };
 
 static int srv_err(lvbgs_server* s, int code, const char* msg) {
@@ -75,8 +80,13 @@ static int srv_err(lvbgs_server* s, int code, const char* msg) {
     return code;
 }
 
+// grid tables of a step: group g, kind k at row g * kNumGrids + k, max_batch + 1 entries per row
+static size_t table_words(int max_batch) {
+    return (size_t)lvbgs_server::kMaxGroups * kNumGrids * (size_t)(max_batch + 1);
+}
+
 static size_t stage_bytes(int max_batch) {
-    return sizeof(View) * (size_t)max_batch + sizeof(uint32_t) * (size_t)kNumGrids * (max_batch + 1);
+    return sizeof(View) * (size_t)max_batch + sizeof(uint32_t) * table_words(max_batch);
 }
 
 extern "C" int lvbgs_server_default_options(lvbgs_server_options* o) {
@@ -101,6 +111,11 @@ static void server_release(lvbgs_server* s) {
     for (auto e : s->ev_free) (void)hipEventDestroy(e);
     for (auto& q : s->queued) (void)hipEventDestroy(q.done);
     for (auto e : s->init_ev) (void)hipEventDestroy(e);
+    for (int g = 1; g < lvbgs_server::kMaxGroups; ++g)
+        if (s->gstream[g]) (void)hipStreamDestroy(s->gstream[g]);
+    for (auto e : s->ev_join)
+        if (e) (void)hipEventDestroy(e);
+    if (s->ev_fork) (void)hipEventDestroy(s->ev_fork);
     if (s->stream) (void)hipStreamDestroy(s->stream);
 }
 
@@ -114,7 +129,7 @@ extern "C" int lvbgs_server_create(int device, const lvbgs_server_options* opts,
     if (o.max_inflight < 1 || o.max_inflight > 16 || o.max_batch < 1 || o.max_batch > 4096 || L.seq_limit < 1 ||
         L.seq_limit > 255 || L.cont_blocks < 1 || L.fixpoint_launches < 0 || L.fixpoint_launches > 64 ||
         L.fixpoint_blocks < 1 || L.cont_px_per_block < 0 || L.fix_px_per_block < 0 ||
-        (L.fetch_depth != 2 && L.fetch_depth != 4))
+        (L.fetch_depth != 2 && L.fetch_depth != 4) || L.groups < 1 || L.groups > lvbgs_server::kMaxGroups)
         return LVBGS_ERR_ARG;
     if (hipSetDevice(device) != hipSuccess) return LVBGS_ERR_HIP;
     lvbgs_server* s = new lvbgs_server();
@@ -127,9 +142,15 @@ extern "C" int lvbgs_server_create(int device, const lvbgs_server_options* opts,
     s->opt.fixpoint_blocks = L.fixpoint_blocks;
     s->opt.fix_px_per_block = L.fix_px_per_block;
     s->opt.fetch_depth = L.fetch_depth;
+    s->groups = L.groups;
     bool ok = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) == hipSuccess &&
               hipMalloc(&s->d_views, sizeof(View) * o.max_batch) == hipSuccess &&
-              hipMalloc(&s->d_prefix, sizeof(uint32_t) * kNumGrids * (o.max_batch + 1)) == hipSuccess;
+              hipMalloc(&s->d_prefix, sizeof(uint32_t) * table_words(o.max_batch)) == hipSuccess;
+    s->gstream[0] = s->stream;
+    for (int g = 1; ok && g < s->groups; ++g)
+        ok = hipStreamCreateWithFlags(&s->gstream[g], hipStreamNonBlocking) == hipSuccess &&
+             hipEventCreateWithFlags(&s->ev_join[g], hipEventDisableTiming) == hipSuccess;
+    if (ok && s->groups > 1) ok = hipEventCreateWithFlags(&s->ev_fork, hipEventDisableTiming) == hipSuccess;
     const int nstage = o.max_inflight + 1;
     for (int i = 0; ok && i < nstage; ++i) {
         uint8_t* p = nullptr;
@@ -218,39 +239,72 @@ static int launch_step(lvbgs_server* s) {
     uint32_t* hp = (uint32_t*)(s->stage[(size_t)si] + sizeof(View) * (size_t)s->o.max_batch);
     QueuedStep q;
     q.frames.reserve((size_t)n);
-    for (int k = 0; k < kNumGrids; ++k) hp[(size_t)k * (n + 1)] = 0;
+    std::vector<Request> req((size_t)n);
     for (int m = 0; m < n; ++m) {
         Slot& sl = s->slots[(size_t)members[(size_t)m]];
-        const Request r = sl.pending.front();
+        req[(size_t)m] = sl.pending.front();
         sl.pending.pop_front();
         ++sl.inflight;
-        lvbgs_ctx* c = sl.ctx;
-        View v = c->v;
-        v.frame = r.img;
-        v.out_mask = r.mask;
-        v.lr = r.lr;
-        v.seq_limit = (uint32_t)s->opt.seq_limit;
-        hv[m] = v;
-        ++c->apply_count;
-        uint32_t g[kNumGrids];
-        kernel_grids(c->v, s->opt, g);
-        for (int k = 0; k < kNumGrids; ++k)
-            hp[(size_t)k * (n + 1) + m + 1] = hp[(size_t)k * (n + 1) + m] + g[k];
-        q.frames.emplace_back(members[(size_t)m], r.tag);
+        ++sl.ctx->apply_count;
+        q.frames.emplace_back(members[(size_t)m], req[(size_t)m].tag);
+    }
+    // groups: largest frames first, dealt round-robin; a group is a contiguous range of View slots
+    const int G = std::min(s->groups, n);
+    std::vector<int> by_size((size_t)n);
+    for (int m = 0; m < n; ++m) by_size[(size_t)m] = m;
+    std::stable_sort(by_size.begin(), by_size.end(), [&](int x, int y) {
+        return s->slots[(size_t)members[(size_t)x]].ctx->npx > s->slots[(size_t)members[(size_t)y]].ctx->npx;
+    });
+    const size_t row = (size_t)n + 1;
+    int first[lvbgs_server::kMaxGroups] = {}, count[lvbgs_server::kMaxGroups] = {};
+    int pos = 0;
+    for (int g = 0; g < G; ++g) {
+        first[g] = pos;
+        for (int k = 0; k < kNumGrids; ++k) hp[((size_t)g * kNumGrids + k) * row] = 0;
+        for (int j = g; j < n; j += G, ++pos) {
+            const int m = by_size[(size_t)j];
+            const Request& r = req[(size_t)m];
+            lvbgs_ctx* c = s->slots[(size_t)members[(size_t)m]].ctx;
+            View v = c->v;
+            v.frame = r.img;
+            v.out_mask = r.mask;
+            v.lr = r.lr;
+            v.seq_limit = (uint32_t)s->opt.seq_limit;
+            hv[pos] = v;
+            uint32_t gk[kNumGrids];
+            kernel_grids(c->v, s->opt, gk);
+            const int jj = pos - first[g];
+            for (int k = 0; k < kNumGrids; ++k) {
+                uint32_t* t = &hp[((size_t)g * kNumGrids + k) * row];
+                t[jj + 1] = t[jj] + gk[k];
+            }
+        }
+        count[g] = pos - first[g];
     }
     if (hipMemcpyAsync(s->d_views, hv, sizeof(View) * (size_t)n, hipMemcpyHostToDevice, s->stream) != hipSuccess ||
-        hipMemcpyAsync(s->d_prefix, hp, sizeof(uint32_t) * kNumGrids * (size_t)(n + 1), hipMemcpyHostToDevice,
+        hipMemcpyAsync(s->d_prefix, hp, sizeof(uint32_t) * (size_t)G * kNumGrids * row, hipMemcpyHostToDevice,
                        s->stream) != hipSuccess ||
         hipEventRecord(s->stage_ev[(size_t)si], s->stream) != hipSuccess)
         return srv_err(s, LVBGS_ERR_HIP, "step upload");
     s->stage_live[(size_t)si] = true;
-    const uint32_t* prefix[kNumGrids];
-    uint32_t total[kNumGrids];
-    for (int k = 0; k < kNumGrids; ++k) {
-        prefix[k] = s->d_prefix + (size_t)k * (n + 1);
-        total[k] = hp[(size_t)k * (n + 1) + n];
+    if (G > 1) {
+        if (hipEventRecord(s->ev_fork, s->stream) != hipSuccess) return srv_err(s, LVBGS_ERR_HIP, "fork");
+        for (int g = 1; g < G; ++g)
+            if (hipStreamWaitEvent(s->gstream[g], s->ev_fork, 0) != hipSuccess) return srv_err(s, LVBGS_ERR_HIP, "fork");
     }
-    batch_step(s->d_views, n, prefix, total, s->opt, s->stream);
+    for (int g = 0; g < G; ++g) {
+        const uint32_t* prefix[kNumGrids];
+        uint32_t total[kNumGrids];
+        for (int k = 0; k < kNumGrids; ++k) {
+            prefix[k] = s->d_prefix + ((size_t)g * kNumGrids + k) * row;
+            total[k] = hp[((size_t)g * kNumGrids + k) * row + (size_t)count[g]];
+        }
+        batch_step(s->d_views + first[g], count[g], prefix, total, s->opt, s->gstream[g]);
+    }
+    for (int g = 1; g < G; ++g)
+        if (hipEventRecord(s->ev_join[g], s->gstream[g]) != hipSuccess ||
+            hipStreamWaitEvent(s->stream, s->ev_join[g], 0) != hipSuccess)
+            return srv_err(s, LVBGS_ERR_HIP, "join");
     q.done = s->ev_free.back();
     s->ev_free.pop_back();
     if (hipEventRecord(q.done, s->stream) != hipSuccess || hipGetLastError() != hipSuccess) {

@@ -37,14 +37,15 @@ def _open(srv, s):
     return srv.open(s.d[0].data_ptr(), s.w, s.h, s.c, **s.params)
 
 
-def test_serving_mix_matches_each_stream_alone():
+@pytest.mark.parametrize("groups", [1, 3])
+def test_serving_mix_matches_each_stream_alone(groups):
     torch = _torch()
     shapes = [(96, 72, 3), (160, 120, 1), (128, 96, 3), (64, 48, 3), (64, 48, 3)]
     streams = [Stream(torch, i, w, h, c, 14) for i, (w, h, c) in enumerate(shapes)]
     torch.cuda.synchronize()
     sizes = [Server.model_bytes(s.w, s.h, s.c) for s in streams]
     # room for the first three models only
-    srv = Server(hbm_budget=sizes[0] + sizes[1] + sizes[2] + sizes[3] // 2, max_inflight=2)
+    srv = Server(hbm_budget=sizes[0] + sizes[1] + sizes[2] + sizes[3] // 2, max_inflight=2, groups=groups)
     ids = {}
     for i in range(3):
         ids[i] = _open(srv, streams[i])
