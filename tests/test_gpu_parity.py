@@ -93,6 +93,16 @@ def test_subsense_gray_qvga():
     run_sequence(SUBSENSE, seq(n=40, channels=1))
 
 
+def test_subsense_rgb_ragged_with_area_downsample():
+    # 330x246: width not a multiple of 4 (byte paths of the word-wise median/dilation), frame large
+    # enough for learning-rate scaling, so the 8x8 area downsample runs with fractional cell weights
+    run_sequence(SUBSENSE, seq(n=14, width=330, height=246, n_objects=3, obj_min=12, obj_max=50))
+
+
+def test_subsense_gray_ragged_small():
+    run_sequence(SUBSENSE, seq(n=24, width=101, height=77, channels=1, n_objects=2, obj_min=6, obj_max=20))
+
+
 def test_subsense_rgb_vga_5x5_spread():
     # 640x480 > 2*QVGA: 5x5 diffusion and median 13
     run_sequence(SUBSENSE, seq(n=15, width=640, height=480, n_objects=4, obj_min=30, obj_max=80))
