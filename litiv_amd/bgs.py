@@ -422,7 +422,8 @@ class Batch:
 
     def set_options(self, **kw):
         """Launch options (use_graph, seq_limit, cont_blocks, cont_px_per_block, fixpoint_launches,
-        fixpoint_blocks, fix_px_per_block); unspecified ones keep their current value."""
+        fixpoint_blocks, fix_px_per_block, fetch_depth, groups); unspecified ones keep their current
+        value. Options never change results."""
         o = BatchOptions()
         self._check(lib().lvbgs_batch_get_options(self._h, C.byref(o)))
         for k, v in kw.items():
