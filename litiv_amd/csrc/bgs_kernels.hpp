@@ -15,7 +15,7 @@ struct StepOptions {
     int fixpoint_blocks = 256;   // grid of each fix-point kernel (grid-stride over the worklist)
     int tail_cap = 1 << 16;      // iteration cap of the single-workgroup tail (error flag if reached)
     int seq_limit = 255;         // samples scanned per pixel by the sequential phase of the match
-    int fetch_depth = 2;         // sample loads in flight per lane in the sequential phase (2 or 4)
+    int fetch_depth = 4;         // sample loads in flight per lane in the sequential phase (2 or 4)
     int cont_blocks = 1024;      // grid of the cooperative phase (grid-stride over unfinished pixels)
     int fix_px_per_block = 0;    // > 0: fix-point grid of a stream = ceil(npx / this), at most fixpoint_blocks
     int cont_px_per_block = 0;   // > 0: cooperative grid of a stream = ceil(npx / this), at most cont_blocks
