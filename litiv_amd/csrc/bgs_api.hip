@@ -165,6 +165,9 @@ static void carve(lvbgs_ctx* c, Arena& ar) {
     v.wl0 = ar.take<uint32_t>(npx);
     v.wl1 = ar.take<uint32_t>(npx);
     v.cont = ar.take<uint32_t>(npx);
+    // cached contexts for up to an eighth of the pixels; entries beyond rebuild theirs
+    v.cont_ctx_cap = (uint32_t)std::min(npx, std::max(npx / 8, (size_t)1024));
+    v.cont_ctx = ar.take<uint32_t>((size_t)v.cont_ctx_cap * kCtxWords);
     v.cont2 = ar.take<uint32_t>(npx);
     v.stamp = ar.take<unsigned long long>(npx);
     v.dsLT = ar.take<float>(nds);

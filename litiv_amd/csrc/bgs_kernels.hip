@@ -258,11 +258,13 @@ LV_D void subsense_thresholds(const View& v, float R, uint32_t unst_old, PxCtx<C
         col /= 2u;
         px.t_col = clamp_u32(col);
         px.t_desc = clamp_u32(desc);
+        px.t_sc = px.t_tot_col = px.t_tot_desc = 0u;  // unused by the 1-channel test
     } else {
         const uint64_t tot_col = col * 3u, tot_desc = desc * 3u;
         px.t_sc = clamp_u32(tot_col / 2u);
         px.t_tot_col = clamp_u32(tot_col);
         px.t_tot_desc = clamp_u32(tot_desc);
+        px.t_col = px.t_desc = 0u;  // unused by the 3-channel test
     }
 }
 
@@ -272,6 +274,7 @@ LV_D void lobster_thresholds(const View& v, PxCtx<C>& px) {
     if constexpr (C == 1) {
         px.t_col = v.color_thr / 2u;
         px.t_desc = v.desc_thr;
+        px.t_sc = px.t_tot_col = px.t_tot_desc = 0u;  // unused by the 1-channel test
     } else {
         px.t_tot_desc = v.desc_thr * 3u;
         px.t_tot_col = v.color_thr * 3u;
@@ -796,6 +799,62 @@ LV_D void build_ctx_staged(const View& v, const volatile uint8_t* win, const PxI
     else lobster_thresholds<C>(v, px);
 }
 
+// A match context in kCtxWords words (16-byte aligned): taps (12 words), packed centre colours and
+// intra descriptors (3), thresholds (5). Written by the speculative match for the pixels it leaves
+// to the continuation, which then reads it instead of rebuilding it from the frame.
+template <int C>
+LV_D void store_ctx(uint32_t* d, const PxCtx<C>& px) {
+    uint32_t w[kCtxWords] = {};
+#pragma unroll
+    for (int c = 0; c < C; ++c)
+#pragma unroll
+        for (int k = 0; k < 4; ++k) w[c * 4 + k] = px.taps[c][k];
+    uint32_t cur = 0;
+#pragma unroll
+    for (int c = 0; c < C; ++c) cur |= px.cur[c] << (8 * c);
+    w[12] = cur;
+    w[13] = px.intra[0] | (C == 3 ? px.intra[1 % C] << 16 : 0u);
+    w[14] = C == 3 ? px.intra[2 % C] : 0u;
+    w[15] = px.t_sc;
+    w[16] = px.t_tot_col;
+    w[17] = px.t_tot_desc;
+    w[18] = px.t_col;
+    w[19] = px.t_desc;
+    uint4* q = (uint4*)d;
+#pragma unroll
+    for (int k = 0; k < kCtxWords / 4; ++k) q[k] = make_uint4(w[4 * k], w[4 * k + 1], w[4 * k + 2], w[4 * k + 3]);
+}
+
+template <int C>
+LV_D void load_ctx(const uint32_t* d, PxCtx<C>& px) {
+    uint32_t w[kCtxWords];
+    const uint4* q = (const uint4*)d;
+#pragma unroll
+    for (int k = 0; k < kCtxWords / 4; ++k) {
+        const uint4 u = q[k];
+        w[4 * k] = u.x;
+        w[4 * k + 1] = u.y;
+        w[4 * k + 2] = u.z;
+        w[4 * k + 3] = u.w;
+    }
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) px.taps[c][k] = w[c * 4 + k];
+        px.cur[c] = (w[12] >> (8 * c)) & 255u;
+    }
+    px.intra[0] = w[13] & 0xFFFFu;
+    if constexpr (C == 3) {
+        px.intra[1 % C] = w[13] >> 16;
+        px.intra[2 % C] = w[14];
+    }
+    px.t_sc = w[15];
+    px.t_tot_col = w[16];
+    px.t_tot_desc = w[17];
+    px.t_col = w[18];
+    px.t_desc = w[19];
+}
+
 // the same context with the taps read from the frame in global memory
 template <int C, int ALGO>
 LV_D void build_ctx_global(const View& v, const uint8_t* frame, int x, int y, PxCtx<C>& px) {
@@ -837,7 +896,7 @@ struct PredTile;
 template <int C, int ALGO, int KF>
 LV_D bool match_pixel(const View& v, const uint8_t* frame, const uint8_t* lut, const FrameTile<C>& tile,
                       const PredTile& pred, const PxIn<C>& in, int lx, int ly, int x, int y, uint32_t p,
-                      uint32_t seq_limit, uint32_t& examined);
+                      uint32_t seq_limit, uint32_t& examined, PxCtx<C>& px);
 
 // records and previous raw decisions of the raster-earlier part of a tile's 5x5 neighbourhood:
 // rows y0-2 .. y0+TH-1, columns x0-2 .. x0+TW+1
@@ -917,16 +976,21 @@ LV_D void match_body(const View& v, uint32_t bid, uint32_t nblk) {
     uint32_t examined = 0;
     bool cont = false;
     const PxIn<C> in = s_in[src];
+    PxCtx<C> px;
     if (x < v.W && y < v.H && in.roi)
-        cont = !match_pixel<C, ALGO, KF>(v, frame, lut, tile, pred, in, lx, ly, x, y, p, seq_limit, examined);
-    // unfinished pixels go to the cooperative phase (one atomic per wavefront)
+        cont = !match_pixel<C, ALGO, KF>(v, frame, lut, tile, pred, in, lx, ly, x, y, p, seq_limit, examined, px);
+    // unfinished pixels go to the continuation with their context (one atomic per wavefront)
     const unsigned long long bal = __ballot(cont);
     if (bal) {
         const int lane = (int)(threadIdx.x & 63u), leader = __ffsll((long long)bal) - 1;
         uint32_t base = 0;
         if (lane == leader) base = atomicAdd(&v.sc->cont_count, (uint32_t)__popcll(bal));
         base = (uint32_t)__shfl((int)base, leader, 64);
-        if (cont) v.cont[base + (uint32_t)__popcll(bal & ((1ull << lane) - 1ull))] = p;
+        const uint32_t slot = base + (uint32_t)__popcll(bal & ((1ull << lane) - 1ull));
+        if (cont) {
+            v.cont[slot] = p;
+            if (slot < v.cont_ctx_cap) store_ctx<C>(v.cont_ctx + (size_t)slot * kCtxWords, px);
+        }
     }
     block_add<TPB>(examined, v.sc->spec_part);
 }
@@ -936,8 +1000,7 @@ LV_D void match_body(const View& v, uint32_t bid, uint32_t nblk) {
 template <int C, int ALGO, int KF>
 LV_D bool match_pixel(const View& v, const uint8_t* frame, const uint8_t* lut, const FrameTile<C>& tile,
                       const PredTile& pred, const PxIn<C>& in, int lx, int ly, int x, int y, uint32_t p,
-                      uint32_t seq_limit, uint32_t& examined) {
-    PxCtx<C> px;
+                      uint32_t seq_limit, uint32_t& examined, PxCtx<C>& px) {
     build_ctx_staged<C, ALGO, FrameTile<C>::LW * C>(v, &tile.px[(ly * FrameTile<C>::LW + lx) * C], in, px);
     // guess: raster-earlier writers decide as in the previous frame
     const uint32_t applied = applied_writers_tile(pred, lx, ly);
@@ -1007,7 +1070,8 @@ LV_D void match_seq_body(const View& v, uint32_t bid, uint32_t nblk) {
         const uint32_t p = v.cont[i];
         const int x = (int)(p % (uint32_t)v.W), y = (int)(p / (uint32_t)v.W);
         PxCtx<C> px;
-        build_ctx_global<C, ALGO>(v, frame, x, y, px);
+        if (i < v.cont_ctx_cap) load_ctx<C>(v.cont_ctx + (size_t)i * kCtxWords, px);
+        else build_ctx_global<C, ALGO>(v, frame, x, y, px);
         const uint32_t applied = applied_writers_global<false>(v, x, y);
         uint64_t sbm;
         int nrep;

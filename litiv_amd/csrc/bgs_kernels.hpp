@@ -9,6 +9,7 @@ namespace lvbgs {
 // largest sequential-match fetch depth; the sample planes are padded by this many samples so a
 // scan may load past a pixel's last sample
 constexpr int kMaxFetch = 4;
+constexpr int kCtxWords = 20;  // words of a cached match context
 
 struct StepOptions {
     int fixpoint_launches = 3;   // fix-point iterations launched as full-grid kernels

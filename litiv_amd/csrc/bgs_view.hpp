@@ -50,6 +50,8 @@ struct View {
     uint32_t* wl0;           // worklists (n_roi entries each)
     uint32_t* wl1;
     uint32_t* cont;          // pixels left by the speculative match to the continuation (n_roi entries)
+    uint32_t* cont_ctx;      // match contexts of the first cont_ctx_cap entries of `cont` (kCtxWords each)
+    uint32_t cont_ctx_cap;
     uint32_t* cont2;         // pixels left by the continuation to the cooperative phase
     unsigned long long* stamp;  // W*H worklist de-duplication stamps
     float *dsLT, *dsST;      // (W/8)*(H/8)*C
