@@ -143,6 +143,26 @@ def test_neighbour_tables_cover_the_ring(fn, n, radius):
         assert 2 <= x <= 7 and 2 <= y <= 5
 
 
+def test_pow_square_compare_is_an_exact_product():
+    """The R feedback compare `R < std::pow(1.0f + 2*min(Dlt, Dst), 2)` (BackgroundSubtractorSuBSENSE.cpp:467)
+    promotes the float base to double and calls libm pow; the oracle and the kernel compare against
+    (double)base * (double)base instead. Every float32 base the compare can see (min of two maps kept in
+    [0, 1], so base in [1, 3]) is enumerated and libm's pow(base, 2) -- math.pow calls it directly -- must
+    equal the double product bit for bit; the two compares then agree for every R."""
+    import itertools
+    import math
+    lo = int(np.float32(1.0).view(np.uint32))
+    hi = int(np.float32(3.0).view(np.uint32))
+    step = 1 << 20
+    for start in range(lo, hi + 1, step):
+        bits = np.arange(start, min(start + step, hi + 1), dtype=np.uint32)
+        base = bits.view(np.float32).astype(np.float64)
+        prod = base * base
+        ref = np.fromiter(map(math.pow, base.tolist(), itertools.repeat(2.0)), np.float64, count=base.size)
+        bad = np.flatnonzero(ref != prod)
+        assert bad.size == 0, "pow(%r, 2) = %r, product %r" % (base[bad[0]], ref[bad[0]], prod[bad[0]])
+
+
 def test_l1dist_uchar_wraps():
     a = np.array([200, 200, 200], np.uint8)
     b = np.zeros(3, np.uint8)
