@@ -2007,39 +2007,48 @@ LV_D void ds_update_cell(const View& v, uint32_t cell, const uint8_t m[C], uint3
     contrib = C == 1 ? best / 2u : best;
 }
 
-// resize(INTER_AREA, W/8 x H/8) + accumulateWeighted LT/ST + per-cell |ST-LT| (SUB:568-583).
-// Frame sizes that are multiples of 8: one thread per pixel column of a row of 32 cells, 8 rows
-// summed per thread, the 8 columns of a cell reduced across lanes; other sizes: one thread per cell.
+// channel sums of one cell row: the 8 pixels (8*C bytes) at `row`, read as C aligned 8-byte words
+template <int C>
+LV_D void cell_row_sum(const uint8_t* row, uint32_t sum[C]) {
+    const uint2* q = (const uint2*)row;
+#pragma unroll
+    for (int j = 0; j < C; ++j) {
+        const uint2 w = q[j];
+#pragma unroll
+        for (int b = 0; b < 8; ++b) sum[(8 * j + b) % C] += ((b < 4 ? w.x : w.y) >> (8 * (b & 3))) & 0xFFu;
+    }
+}
+
+// resize(INTER_AREA, W/8 x H/8) + accumulateWeighted LT/ST + per-cell |ST-LT| (SUB:568-583), one
+// thread per cell in raster order. Frame sizes that are multiples of 8 (an integer factor: the
+// rounded 8x8 block mean) read each cell row as C 8-byte words (bytes when the frame is not 8-byte
+// aligned), so a wavefront's loads cover contiguous runs of 64 cells; other sizes take the general
+// area weights.
 template <int C>
 LV_D void downsample_body(const View& v, uint32_t bid, uint32_t nblk) {
     const uint8_t* frame = v.frame;
     uint32_t contrib = 0;
-    if (ds_exact8(v)) {
-        const int nbx = (v.dsW + 31) / 32;
-        const int cy = (int)bid / nbx, cx = ((int)bid % nbx) * 32 + (int)threadIdx.x / 8;
-        const bool ok = cy < v.dsH && cx < v.dsW;
-        uint32_t sum[C];
-        for (int c = 0; c < C; ++c) sum[c] = 0;
-        if (ok) {
-            const int x = cx * 8 + (int)(threadIdx.x & 7u);
-            for (int r = 0; r < 8; ++r) {
-                const uint8_t* px = frame + ((size_t)(cy * 8 + r) * v.W + x) * C;
-                for (int c = 0; c < C; ++c) sum[c] += px[c];
-            }
-        }
-        for (int c = 0; c < C; ++c)
-            for (int o = 1; o < 8; o <<= 1) sum[c] += (uint32_t)__shfl_xor((int)sum[c], o, 64);
-        if (ok && (threadIdx.x & 7u) == 0) {
-            uint8_t m[C];
-            for (int c = 0; c < C; ++c) m[c] = sat_u8((float)sum[c] * (1.f / 64.f));
-            ds_update_cell<C>(v, (uint32_t)(cy * v.dsW + cx), m, contrib);
-        }
-        block_add<TPB>(contrib, v.sc->cd_part);
-        return;
-    }
-    const uint32_t cell = bid * TPB + threadIdx.x;
     const uint32_t ncell = (uint32_t)(v.dsW * v.dsH);
-    if (cell < ncell) {
+    const uint32_t cell = bid * TPB + threadIdx.x;
+    if (cell < ncell && ds_exact8(v)) {
+        const int dx = (int)(cell % (uint32_t)v.dsW), dy = (int)(cell / (uint32_t)v.dsW);
+        uint32_t sum[C];
+#pragma unroll
+        for (int c = 0; c < C; ++c) sum[c] = 0;
+        const uint8_t* cell0 = frame + ((size_t)dy * 8 * v.W + (size_t)dx * 8) * C;
+        if (((uintptr_t)frame & 7u) == 0) {
+#pragma unroll
+            for (int r = 0; r < 8; ++r) cell_row_sum<C>(cell0 + (size_t)r * v.W * C, sum);
+        } else {
+            for (int r = 0; r < 8; ++r)
+#pragma unroll
+                for (int b = 0; b < 8 * C; ++b) sum[b % C] += cell0[(size_t)r * v.W * C + b];
+        }
+        uint8_t mc[C];
+#pragma unroll
+        for (int c = 0; c < C; ++c) mc[c] = sat_u8((float)sum[c] * (1.f / 64.f));
+        ds_update_cell<C>(v, cell, mc, contrib);
+    } else if (cell < ncell) {
         const int dx = (int)(cell % (uint32_t)v.dsW), dy = (int)(cell / (uint32_t)v.dsW);
         uint8_t mc[C];
         for (int c = 0; c < C; ++c) {
@@ -2406,9 +2415,7 @@ void kernel_grids(const View& v, const StepOptions& opt, uint32_t g[kNumGrids]) 
     g[kGridPost] = cdiv(v.W, PW) * cdiv(v.H, PH);
     g[kGridZone] = cdiv((size_t)cdiv(v.W, ZS) * cdiv(v.H, ZS), ZPB);
     g[kGridPix] = cdiv(v.npx, TPB);
-    g[kGridCell] = !v.lr_scaling ? 0u
-                   : (v.W == 8 * v.dsW && v.H == 8 * v.dsH) ? cdiv(v.dsW, 32) * (unsigned)v.dsH
-                                                               : cdiv((size_t)v.dsW * v.dsH, TPB);
+    g[kGridCell] = !v.lr_scaling ? 0u : cdiv((size_t)v.dsW * v.dsH, TPB);
     g[kGridOne] = 1u;
     g[kGridFix] = (uint32_t)opt.fixpoint_blocks;
     if (opt.fix_px_per_block > 0) g[kGridFix] = std::min(g[kGridFix], cdiv(v.npx, (size_t)opt.fix_px_per_block));

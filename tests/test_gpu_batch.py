@@ -11,7 +11,7 @@ pytestmark = pytest.mark.gpu
 
 STATE = ["samples_color", "samples_desc", "last_color", "last_desc", "last_fg", "T", "R", "V", "MLD", "DminLT",
          "DminST", "RawLT", "RawST", "FinLT", "FinST", "unstable", "blinks", "last_raw", "last_raw_blink",
-         "last_dil_inv"]
+         "last_dil_inv", "dsLT", "dsST"]
 
 
 def _torch():

@@ -2,7 +2,7 @@
 
 Bar: masks bit-exact on every frame; the whole model state (samples, FP32 feedback maps, u8 maps,
 LUT, frame-level scalars) bit-exact after the sequence. Sizes are chosen so the oracle finishes in
-seconds; full-size properties are in test_gpu_fullsize.py.
+seconds; the full-size cases are in test_gpu_fullsize.py.
 """
 import numpy as np
 import pytest
@@ -14,7 +14,7 @@ pytestmark = pytest.mark.gpu
 
 SUB_STATE = ["samples_color", "samples_desc", "last_color", "last_desc", "last_fg", "lut", "T", "R", "V", "MLD",
              "DminLT", "DminST", "RawLT", "RawST", "FinLT", "FinST", "unstable", "blinks", "last_raw",
-             "last_raw_blink", "last_dil_inv"]
+             "last_raw_blink", "last_dil_inv", "dsLT", "dsST"]
 LOB_STATE = ["samples_color", "samples_desc", "last_color", "last_fg", "lut"]
 
 
