@@ -150,8 +150,9 @@ int lvbgs_take_timing(lvbgs_ctx* ctx, double* total_ms, int* count);
 /* Fix-point schedule: full-grid iterations before the single-workgroup tail, and their grid size. */
 int lvbgs_set_fixpoint_schedule(lvbgs_ctx* ctx, int launches, int blocks);
 
-/* Match schedule: samples scanned per pixel by the sequential phase (1..255) before unfinished pixels
- * move to the cooperative phase, and the grid of that phase. Results do not depend on it. */
+/* Match schedule: samples scanned per pixel by the sequential phase (1..255; 0, the default: 8 for
+ * SuBSENSE RGB frames of 1280x720 pixels or more, else 255) before unfinished pixels move to the
+ * continuation phases, and the grid of those phases. Results do not depend on it. */
 int lvbgs_set_match_schedule(lvbgs_ctx* ctx, int seq_limit, int cont_blocks);
 
 /* RNG self-check: the four draws of group 0 of counter (pixel p, frame, phase, event) for p in [0, n),

@@ -355,6 +355,9 @@ class BackgroundSubtractor:
         self._check(lib().lvbgs_set_fixpoint_schedule(self._h, int(launches), int(blocks)))
 
     def set_match_schedule(self, seq_limit, cont_blocks=1024):
+        """Samples scanned per pixel by the sequential match phase (1..255; 0 = automatic: 8 for
+        SuBSENSE RGB frames of 1280x720 or more, else 255) and the continuation grid; results do
+        not depend on it."""
         self._check(lib().lvbgs_set_match_schedule(self._h, int(seq_limit), int(cont_blocks)))
 
     def debug_rng(self, frame, phase, event, n):

@@ -125,7 +125,12 @@ LV_HD void neighbor_pos(uint32_t rnd, bool use3, int x, int y, int w, int h, int
 // ------------------------------------------------------------------------------------------
 constexpr int kWlRing = 4;
 constexpr int kStripes = 64;
-struct alignas(16) Scalars {
+// The match continuation list is split into kContStripes stripes: tile b appends to stripe
+// b % kContStripes, each stripe's counter on its own 128-byte line, so appends from different
+// tiles do not serialize on one address.
+constexpr int kContStripes = 64;
+constexpr int kContStride = 32;  // u32 words between stripe counters
+struct alignas(128) Scalars {
     uint32_t frame_idx;          // frames applied since initialize (SuBSENSE m_nFrameIdx)
     uint32_t frames_since_reset; // m_nFramesSinceLastReset
     uint32_t cooldown;           // m_nModelResetCooldown
@@ -155,9 +160,10 @@ struct alignas(16) Scalars {
     uint32_t max_iter_seen;      // largest fix-point iteration index that had work (sticky)
     uint32_t epoch;              // steps since allocation (worklist de-duplication tags)
     uint32_t wl_work[8];         // per-frame: worklist size of fix-point iterations 1..7
-    uint32_t cont_count;         // per-frame: pixels handed to the match continuation
     uint32_t cont2_count;        // per-frame: pixels handed to the cooperative match phase
     uint8_t lut[256];            // LBSP threshold LUT (m_anLBSPThreshold_8bitLUT)
+    // per-frame: pixels handed to the match continuation, stripe k at cont_cnt[k * kContStride]
+    alignas(128) uint32_t cont_cnt[kContStripes * kContStride];
 };
 constexpr uint32_t kErrFixpointCap = 1u;
 

@@ -632,11 +632,11 @@ LV_D void describe_body(const View& v, uint32_t bid, uint32_t nblk) {
     if (bid == 0 && threadIdx.x == 0) {
         // per-frame counters consumed by later kernels of this step
         for (int i = 0; i < kWlRing; ++i) v.sc->wl_count[i] = 0;
-        v.sc->cont_count = 0;
         v.sc->cont2_count = 0;
         for (int i = 0; i < 8; ++i) v.sc->wl_work[i] = 0;
         v.sc->iters = 0;
     }
+    if (bid == 0 && threadIdx.x < kContStripes) v.sc->cont_cnt[threadIdx.x * kContStride] = 0;
     if (bid == 0 && threadIdx.x < kStripes) {
         v.sc->ex_part[threadIdx.x] = 0;
         v.sc->wr_part[threadIdx.x] = 0;
@@ -979,17 +979,20 @@ LV_D void match_body(const View& v, uint32_t bid, uint32_t nblk) {
     PxCtx<C> px;
     if (x < v.W && y < v.H && in.roi)
         cont = !match_pixel<C, ALGO, KF>(v, frame, lut, tile, pred, in, lx, ly, x, y, p, seq_limit, examined, px);
-    // unfinished pixels go to the continuation with their context (one atomic per wavefront)
+    // unfinished pixels go to the continuation with their context: one atomic per wavefront, on
+    // this tile's stripe of the list (at most cont_region entries reach one stripe)
     const unsigned long long bal = __ballot(cont);
     if (bal) {
         const int lane = (int)(threadIdx.x & 63u), leader = __ffsll((long long)bal) - 1;
+        const uint32_t k = bid % (uint32_t)kContStripes;
         uint32_t base = 0;
-        if (lane == leader) base = atomicAdd(&v.sc->cont_count, (uint32_t)__popcll(bal));
+        if (lane == leader) base = atomicAdd(&v.sc->cont_cnt[k * kContStride], (uint32_t)__popcll(bal));
         base = (uint32_t)__shfl((int)base, leader, 64);
-        const uint32_t slot = base + (uint32_t)__popcll(bal & ((1ull << lane) - 1ull));
+        const uint32_t j = base + (uint32_t)__popcll(bal & ((1ull << lane) - 1ull));
         if (cont) {
-            v.cont[slot] = p;
-            if (slot < v.cont_ctx_cap) store_ctx<C>(v.cont_ctx + (size_t)slot * kCtxWords, px);
+            v.cont[(size_t)k * v.cont_region + j] = p;
+            if (j < v.cont_ctx_region)
+                store_ctx<C>(v.cont_ctx + ((size_t)k * v.cont_ctx_region + j) * kCtxWords, px);
         }
     }
     block_add<TPB>(examined, v.sc->spec_part);
@@ -1053,24 +1056,42 @@ LV_D void finish_pixel(const View& v, uint32_t p, int x, int y, const ScanState&
 }
 
 // Phase B: the scans phase A cut at its sample limit, continued with one lane per pixel over the
-// dense list of unfinished pixels, so long scans (mostly foreground pixels, which examine every
-// sample) fill whole wavefronts. The context is rebuilt from global memory; a pixel with more than
-// kRepl overwritten slots goes on to phase C.
+// dense list of unfinished pixels (its stripes walked in order), so long scans (mostly foreground
+// pixels, which examine every sample) fill whole wavefronts. The context is the one phase A cached
+// (rebuilt from global memory past a stripe's cache); a pixel with more than kRepl overwritten
+// slots goes on to phase C.
 template <int C, int ALGO, int KF>
 LV_D void match_seq_body(const View& v, uint32_t bid, uint32_t nblk) {
+    static_assert(kContStripes <= 64 && (kContStripes & (kContStripes - 1)) == 0, "one wavefront scans the stripe counts");
     const uint8_t* frame = v.frame;
     __shared__ uint8_t lut[256];
-    const uint32_t n = v.sc->cont_count;
-    if (bid * TPB >= n) return;  // uniform: no item reaches this block
+    __shared__ uint32_t s_pre[kContStripes + 1];  // exclusive prefix of the stripe counts
+    if (threadIdx.x < 64u) {
+        const int lane = (int)threadIdx.x;
+        uint32_t c = lane < kContStripes ? v.sc->cont_cnt[lane * kContStride] : 0u;
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t t = (uint32_t)__shfl_up((int)c, (unsigned)o, 64);
+            if (lane >= o) c += t;
+        }
+        if (lane < kContStripes) s_pre[lane + 1] = c;
+        if (lane == 0) s_pre[0] = 0u;
+    }
     load_lut(lut, v.sc);
     __syncthreads();
+    const uint32_t n = s_pre[kContStripes];
+    if (bid * TPB >= n) return;  // uniform: no item reaches this block
     const uint64_t epoch = v.sc->epoch;
     uint32_t examined = 0;
     for (uint32_t i = bid * TPB + threadIdx.x; i < n; i += nblk * TPB) {
-        const uint32_t p = v.cont[i];
+        // item i = entry j of stripe k: the last stripe whose prefix is at most i
+        uint32_t k = 0;
+        for (uint32_t half = kContStripes / 2; half > 0; half >>= 1)
+            if (s_pre[k + half] <= i) k += half;
+        const uint32_t j = i - s_pre[k];
+        const uint32_t p = v.cont[(size_t)k * v.cont_region + j];
         const int x = (int)(p % (uint32_t)v.W), y = (int)(p / (uint32_t)v.W);
         PxCtx<C> px;
-        if (i < v.cont_ctx_cap) load_ctx<C>(v.cont_ctx + (size_t)i * kCtxWords, px);
+        if (j < v.cont_ctx_region) load_ctx<C>(v.cont_ctx + ((size_t)k * v.cont_ctx_region + j) * kCtxWords, px);
         else build_ctx_global<C, ALGO>(v, frame, x, y, px);
         const uint32_t applied = applied_writers_global<false>(v, x, y);
         uint64_t sbm;
@@ -2409,6 +2430,11 @@ __global__ void __launch_bounds__(TPB) kb_refresh(BatchArgs b) {
 // host-side launchers
 // ==============================================================================================
 static inline unsigned cdiv(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }
+
+uint32_t cont_stripe_slots(int W, int H) {
+    const size_t tiles = (size_t)cdiv(W, TW) * cdiv(H, TH);
+    return (uint32_t)((tiles + kContStripes - 1) / kContStripes * TPB);
+}
 
 void kernel_grids(const View& v, const StepOptions& opt, uint32_t g[kNumGrids]) {
     g[kGridTile] = cdiv(v.W, TW) * cdiv(v.H, TH);

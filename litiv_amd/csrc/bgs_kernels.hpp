@@ -12,10 +12,11 @@ constexpr int kMaxFetch = 4;
 constexpr int kCtxWords = 20;  // words of a cached match context
 
 struct StepOptions {
-    int fixpoint_launches = 3;   // fix-point iterations launched as full-grid kernels
+    int fixpoint_launches = 2;   // fix-point iterations launched as full-grid kernels
     int fixpoint_blocks = 256;   // grid of each fix-point kernel (grid-stride over the worklist)
     int tail_cap = 1 << 16;      // iteration cap of the single-workgroup tail (error flag if reached)
-    int seq_limit = 255;         // samples scanned per pixel by the sequential phase of the match
+    int seq_limit = 0;           // samples scanned per pixel by the sequential phase of the match
+                                 // (0: the single-stream path's automatic choice)
     int fetch_depth = 4;         // sample loads in flight per lane in the sequential phase (2 or 4)
     int cont_blocks = 1024;      // grid of the cooperative phase (grid-stride over unfinished pixels)
     int fix_px_per_block = 0;    // > 0: fix-point grid of a stream = ceil(npx / this), at most fixpoint_blocks
@@ -38,6 +39,9 @@ enum GridKind : int {
     kNumGrids
 };
 void kernel_grids(const View& v, const StepOptions& opt, uint32_t g[kNumGrids]);
+// entries of one stripe of the match continuation list of a W x H stream: the most that the tiles
+// appending to one stripe can hand over
+uint32_t cont_stripe_slots(int W, int H);
 
 // one apply() step; v.frame is device memory (W*H*C), v.out_mask optional device W*H
 void frame_step(const View& v, const StepOptions& opt, hipStream_t st);

@@ -49,9 +49,11 @@ struct View {
     uint8_t* dec;            // W*H raw decisions (1 = foreground)
     uint32_t* wl0;           // worklists (n_roi entries each)
     uint32_t* wl1;
-    uint32_t* cont;          // pixels left by the speculative match to the continuation (n_roi entries)
-    uint32_t* cont_ctx;      // match contexts of the first cont_ctx_cap entries of `cont` (kCtxWords each)
-    uint32_t cont_ctx_cap;
+    uint32_t* cont;          // pixels left by the speculative match to the continuation: stripe k's
+                             // entries at k * cont_region (kContStripes stripes)
+    uint32_t* cont_ctx;      // match contexts of the first cont_ctx_region entries of each stripe
+                             // (kCtxWords each, stripe k at k * cont_ctx_region)
+    uint32_t cont_region, cont_ctx_region;
     uint32_t* cont2;         // pixels left by the continuation to the cooperative phase
     unsigned long long* stamp;  // W*H worklist de-duplication stamps
     float *dsLT, *dsST;      // (W/8)*(H/8)*C
