@@ -26,11 +26,12 @@ frames favour a full sequential scan, settled models a short one).
 import numpy as np
 
 SHARED_CANDIDATES = [
+    {"fixpoint_launches": 2, "fetch_depth": 4, "groups": 3},
     {"fixpoint_launches": 2, "fetch_depth": 4, "groups": 2},
     {"fixpoint_launches": 2, "fetch_depth": 4, "groups": 1},
     {"fixpoint_launches": 2, "fetch_depth": 4, "groups": 4},
-    {"fixpoint_launches": 3, "fetch_depth": 4, "groups": 2},
-    {"fixpoint_launches": 2, "fetch_depth": 2, "groups": 2},
+    {"fixpoint_launches": 3, "fetch_depth": 4, "groups": 3},
+    {"fixpoint_launches": 2, "fetch_depth": 2, "groups": 3},
 ]
 CLASS_CHOICES = {"seq_limit": [4, 6, 8, 12, 16, 24, 255], "cont_px_per_block": [512, 1024, 2048, 4096, 8192]}
 CLASS_START = {"seq_limit": 12, "cont_px_per_block": 2048}

@@ -6,7 +6,7 @@ from litiv_amd.autotune import CLASS_CHOICES, OnlineTuner, tune_batch
 
 
 class SimBatch:
-    """Step time = 8 ms + penalties: fetch depth 2 costs 0.2 ms, 2 / 4 groups save 0.15 / 0.3 ms;
+    """Step time = 8 ms + penalties: fetch depth 2 costs 0.2 ms, 2 / 3 / 4 groups save 0.15 / 0.2 / 0.3 ms;
     class 0's seq_limit is best at 8,
     class 1's at 16 (0.12 ms, 1.5%, per step of distance in the choice list, above the tuner's 1%
     switching margin); cont_px_per_block has no effect; Gaussian noise of 0.01 ms."""
@@ -43,7 +43,7 @@ class SimBatch:
         else:
             self.direct_steps += 1
         lims = CLASS_CHOICES["seq_limit"]
-        t = 8.0 + (0.2 if self.o["fetch_depth"] == 2 else 0.0) - {1: 0.0, 2: 0.15, 4: 0.3}[self.o["groups"]]
+        t = 8.0 + (0.2 if self.o["fetch_depth"] == 2 else 0.0) - {1: 0.0, 2: 0.15, 3: 0.2, 4: 0.3}[self.o["groups"]]
         for members, best in (((0, 1), 8), ((2, 3), 16)):
             sl = self.so[members[0]]["seq_limit"] or self.o["seq_limit"]
             t += 0.12 * abs(lims.index(sl) - lims.index(best))
