@@ -141,8 +141,9 @@ int lvbgs_get_scalars(lvbgs_ctx* ctx, double* out, int cap);
 /* hipStream_t of the context (for ordering device-pointer calls with other work). */
 void* lvbgs_stream(lvbgs_ctx* ctx);
 
-/* Kernel timing: stage = "match" | "step" | "" (off). While on, every step records HIP events around
- * the stage on the context's stream; lvbgs_take_timing returns the summed milliseconds and count since
+/* Kernel timing: stage = "match" (phase A of the sample match) | "update" (model update and feedback
+ * maps) | "step" | "" (off). While on, every step records HIP events around the stage on the context's
+ * stream; lvbgs_take_timing returns the summed milliseconds and count since
  * the last take (it synchronizes). */
 int lvbgs_set_timing(lvbgs_ctx* ctx, const char* stage);
 int lvbgs_take_timing(lvbgs_ctx* ctx, double* total_ms, int* count);
@@ -201,7 +202,8 @@ int lvbgs_batch_get_stream_options(const lvbgs_batch* b, int index, int* seq_lim
  * lrs (nullable = 0) its learning-rate override */
 int lvbgs_batch_apply_device(lvbgs_batch* b, const void* const* d_imgs, void* const* d_masks, const double* lrs);
 int lvbgs_batch_synchronize(lvbgs_batch* b);
-/* GPU time of batch steps: stage "match" (the match kernels) | "step" (whole step) | "" (off);
+/* GPU time of batch steps: stage "match" (phase A of the match) | "update" (the update kernel) | "step"
+ * (whole step) | "" (off); kernel stages need groups = 1;
  * HIP events on the batch stream, summed by lvbgs_batch_take_timing (which synchronizes) */
 int lvbgs_batch_set_timing(lvbgs_batch* b, const char* stage);
 int lvbgs_batch_take_timing(lvbgs_batch* b, double* total_ms, int* count);
@@ -223,7 +225,11 @@ typedef struct lvbgs_server_options {
 typedef struct lvbgs_done {
     uint64_t tag;                 /* the submit call's tag */
     int32_t stream;
-    int32_t status;               /* LVBGS_OK */
+    int32_t status;               /* LVBGS_OK, or LVBGS_ERR_DEVICE: a device-side check failed for this
+                                     stream (fix-point iteration cap); the server then closes the stream,
+                                     completes its waiting frames with the same status and rejects
+                                     further submits; lvbgs_server_close releases its id. Other streams
+                                     of the same steps are unaffected. */
 } lvbgs_done;
 typedef struct lvbgs_server_stats {
     uint64_t steps, frames_submitted, frames_launched, frames_done;
@@ -240,7 +246,8 @@ void* lvbgs_server_stream(lvbgs_server* s);
  * frame completes; its frames join steps once the initialization is done. */
 int lvbgs_server_open(lvbgs_server* s, const lvbgs_params* params, const void* d_img0, int width, int height,
                       int channels, int* stream_out);
-/* no further frames; the stream's waiting frames still complete, then its id may be reused */
+/* no further frames; the stream's waiting frames still complete, then its id may be reused.
+ * On a stream the server closed after a device error it acknowledges the failure (LVBGS_OK). */
 int lvbgs_server_close(lvbgs_server* s, int stream);
 int lvbgs_server_submit(lvbgs_server* s, int stream, const void* d_img, void* d_mask, double lr, uint64_t tag);
 /* non-blocking: collects finished steps, launches steps while the device queue has room;
@@ -253,6 +260,29 @@ int lvbgs_server_drain(lvbgs_server* s);
 int lvbgs_server_get_stats(lvbgs_server* s, lvbgs_server_stats* out);
 /* the context of an open stream (getters and state reads after lvbgs_server_drain) */
 lvbgs_ctx* lvbgs_server_context(lvbgs_server* s, int stream);
+/* Test hook (no counterpart in the reference): iteration cap of the single-workgroup fix-point tail
+ * of later steps (default 65536). A stream whose raster-order fix point needs more iterations than
+ * fixpoint_launches + cap stops there with LVBGS_ERR_DEVICE instead of finishing; every kernel still
+ * exits normally. Lets tests drive the failure path from the host. */
+int lvbgs_server_debug_fixpoint_cap(lvbgs_server* s, int tail_cap);
+
+/* ---- evaluation ----
+ * Binary classification counts of a foreground mask against ground truth, on the device:
+ * lv::BinClassif (modules/datasets/include/litiv/datasets/metrics.hpp:32-66) and its accumulate()
+ * (modules/datasets/src/metrics.cpp:21-50). Labels as in the CDnet scripts (metrics.hpp:23-28):
+ * ground truth 85 (out of scope) or 170 (unknown), or a ROI value of 0, make a pixel a don't-care;
+ * otherwise 255 is positive and every other value negative; a positive result on shadow (50)
+ * ground truth also counts as a special error. */
+typedef struct lvbgs_binclassif {
+    uint64_t tp, tn, fp, fn, se, dc;  /* nTP, nTN, nFP, nFN, nSE, nDC */
+} lvbgs_binclassif;
+/* BinClassif::accumulate(oClassif, oGT, oROI): adds the counts of d_classif against d_gt into *counts.
+ * Device pointers, width x height 8-bit masks sharing one row stride in bytes (0 = width); d_gt NULL
+ * counts every pixel as a don't-care (metrics.cpp:26-29), d_roi NULL means no ROI. Ordered on
+ * `stream` (hipStream_t or NULL); synchronizes it before returning. Allocates a small device
+ * buffer per call (about 8 bytes per 1024 pixels). */
+int lvbgs_binclassif_accumulate(const void* d_classif, const void* d_gt, const void* d_roi, int width, int height,
+                                size_t stride, lvbgs_binclassif* counts, void* stream);
 
 #ifdef __cplusplus
 }

@@ -67,6 +67,10 @@ class ServerOptions(C.Structure):
                 ("launch", BatchOptions)]
 
 
+class BinClassifCounts(C.Structure):
+    _fields_ = [(n, C.c_uint64) for n in ("tp", "tn", "fp", "fn", "se", "dc")]
+
+
 class ServerStats(C.Structure):
     _fields_ = [("steps", C.c_uint64), ("frames_submitted", C.c_uint64), ("frames_launched", C.c_uint64),
                 ("frames_done", C.c_uint64), ("bytes_used", C.c_uint64), ("pending_frames", C.c_uint64),
@@ -143,6 +147,8 @@ def lib():
         L.lvbgs_server_get_stats.argtypes = [vp, C.POINTER(ServerStats)]
         L.lvbgs_server_context.argtypes = [vp, i]
         L.lvbgs_server_context.restype = vp
+        L.lvbgs_server_debug_fixpoint_cap.argtypes = [vp, i]
+        L.lvbgs_binclassif_accumulate.argtypes = [vp, vp, vp, i, i, C.c_size_t, C.POINTER(BinClassifCounts), vp]
         L.lvbgs_batch_create.argtypes = [C.POINTER(vp), i, C.POINTER(vp)]
         L.lvbgs_batch_destroy.argtypes = [vp]
         L.lvbgs_batch_last_error.argtypes = [vp]
@@ -478,7 +484,8 @@ class Batch:
         return lib().lvbgs_batch_stream(self._h)
 
     def set_timing(self, stage):
-        """"match" | "step" | "" (off): HIP-event timing of each following step."""
+        """"match" | "update" | "step" | "" (off): HIP-event timing of each following step (kernel stages
+        need groups = 1)."""
         self._check(lib().lvbgs_batch_set_timing(self._h, (stage or "").encode()))
 
     def take_timing(self):
@@ -574,6 +581,11 @@ class Server:
         self._check(lib().lvbgs_server_get_stats(self._h, C.byref(st)))
         return {k: getattr(st, k) for k, _ in ServerStats._fields_}
 
+    def debug_fixpoint_cap(self, tail_cap):
+        """Test hook: iteration cap of the fix-point tail of later steps (default 65536); a stream
+        needing more fails with ERR_DEVICE and is closed by the server."""
+        self._check(lib().lvbgs_server_debug_fixpoint_cap(self._h, int(tail_cap)))
+
     def retired(self, sid):
         """True once a closed stream's last frame has completed (its id may then be reused)."""
         return not lib().lvbgs_server_context(self._h, int(sid))
@@ -584,3 +596,56 @@ class Server:
         if not h:
             raise LvbgsError(ERR_ARG, "unknown stream")
         return BackgroundSubtractorSuBSENSE._borrow(h, self._params[sid], self._shape[sid])
+
+
+class BinClassif:
+    """lv::BinClassif (modules/datasets/include/litiv/datasets/metrics.hpp:32-66): classification
+    counters of binary masks against CDnet-style ground truth, accumulated on the device."""
+
+    FIELDS = ("tp", "tn", "fp", "fn", "se", "dc")
+
+    def __init__(self):
+        self._c = BinClassifCounts()
+
+    def __getattr__(self, name):
+        if name in BinClassif.FIELDS:
+            return getattr(self._c, name)
+        raise AttributeError(name)
+
+    def counts(self):
+        return {k: getattr(self._c, k) for k in self.FIELDS}
+
+    def total(self, with_dont_care=False):
+        c = self._c
+        return c.tp + c.tn + c.fp + c.fn + (c.dc if with_dont_care else 0)
+
+    def accumulate_device(self, d_classif, d_gt, d_roi, width, height, stride=0, stream=None):
+        """accumulate(oClassif, oGT, oROI) (metrics.cpp:21-50) on device masks; d_gt / d_roi may be None."""
+        rc = lib().lvbgs_binclassif_accumulate(C.c_void_p(d_classif), C.c_void_p(d_gt or 0), C.c_void_p(d_roi or 0),
+                                               int(width), int(height), int(stride), C.byref(self._c),
+                                               C.c_void_p(stream or 0))
+        if rc != OK:
+            raise LvbgsError(rc, "lvbgs_binclassif_accumulate failed")
+        return self
+
+    def metrics(self):
+        """BinClassifMetrics (metrics.hpp:203-255) of the accumulated counters."""
+        c = self._c
+        tp, tn, fp, fn = c.tp, c.tn, c.fp, c.fn
+        recall = tp / (tp + fn) if tp + fn > 0 else 0.0
+        precision = tp / (tp + fp) if tp + fp > 0 else 0.0
+        total = tp + tn + fp + fn
+        mcc = 0.0
+        if tp + fp > 0 and tp + fn > 0 and tn + fp > 0 and tn + fn > 0:
+            mcc = (float(tp) * tn - float((fp * fn) % (1 << 64))) / np.sqrt(
+                (float(tp) + fp) * (tp + fn) * (tn + fp) * (tn + fn))
+        return {
+            "recall": recall,
+            "specificity": tn / (tn + fp) if tn + fp > 0 else 0.0,
+            "fpr": fp / (fp + tn) if fp + tn > 0 else 0.0,
+            "fnr": fn / (tp + fn) if tp + fn > 0 else 0.0,
+            "pbc": 100.0 * (fn + fp) / total if total > 0 else 0.0,
+            "precision": precision,
+            "fmeasure": 2.0 * (recall * precision) / (recall + precision) if recall + precision > 0 else 0.0,
+            "mcc": mcc,
+        }

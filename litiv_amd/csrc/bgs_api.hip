@@ -263,13 +263,17 @@ static int initialize(lvbgs_ctx* c, const uint8_t* img, bool device_img, int w, 
     if ((size_t)w * (size_t)h > 0x7FFFFFFFull / 8) return set_err(c, LVBGS_ERR_ARG, "image too large");
     if (hipSetDevice(c->device) != hipSuccess) return set_err(c, LVBGS_ERR_HIP, "hipSetDevice");
     const size_t npx = (size_t)w * h;
-    // ROI (BackgroundSubtractionUtils.cpp:82-101); the last ROI of the same size is reused as is
-    // (:87-88), with the counts it had
+    // ROI (BackgroundSubtractionUtils.cpp:82-101); the last ROI of the same size is reused as
+    // stored (:87-88), already border-trimmed, and recounted (:97-99): both counts are its non-zero
+    // pixels
     const bool reuse_roi = !roi && !c->roi.empty() && c->roi_w == w && c->roi_h == h;
     std::vector<uint8_t> nr;
-    size_t orig = c->orig_roi, fin = c->n_roi;
+    size_t orig = 0, fin = 0;
     if (!reuse_roi) nr.resize(npx);
     if (reuse_roi) {
+        for (uint8_t r : c->roi) orig += r ? 1 : 0;
+        fin = orig;
+        if (!fin) return set_err(c, LVBGS_ERR_ARG, "provided ROI mat contains no useful pixels away from borders");
     } else if (!roi) {
         std::memset(nr.data(), 255, npx);
     } else {
@@ -644,6 +648,8 @@ extern "C" long long lvbgs_get_state(lvbgs_ctx* c, const char* name, void* out, 
         // unpack the sample planes to the oracle layout: N x (H x W x C)
         const size_t rec = C == 3 ? 12 : 4;
         std::vector<uint8_t> a(N * ((npx + 63) / 64 * 64) * rec);
+        // the context's stream does not synchronize with the null stream: wait for its steps
+        if (hipStreamSynchronize(c->stream) != hipSuccess) return LVBGS_ERR_HIP;
         if (hipMemcpy(a.data(), v.sA, a.size(), hipMemcpyDeviceToHost) != hipSuccess) return LVBGS_ERR_HIP;
         const bool col = n == "samples_color";
         bytes = N * npx * C * (col ? 1 : 2);
@@ -710,6 +716,7 @@ extern "C" int lvbgs_set_timing(lvbgs_ctx* c, const char* stage) {
     if (s.empty()) c->timed_stage = 0;
     else if (s == "match") c->timed_stage = 1;
     else if (s == "step") c->timed_stage = 2;
+    else if (s == "update") c->timed_stage = 3;
     else return set_err(c, LVBGS_ERR_ARG, "unknown timing stage");
     c->ev_used = 0;
     return LVBGS_OK;
@@ -1055,8 +1062,8 @@ extern "C" int lvbgs_batch_apply_device(lvbgs_batch* b, const void* const* d_img
     if (b->ev_live[i] && hipEventSynchronize(b->ev_copy[i]) != hipSuccess) return bset_err(b, LVBGS_ERR_HIP, "sync");
     for (int s = 0; s < ns; ++s)
         if (!d_imgs[s]) return bset_err(b, LVBGS_ERR_ARG, "input image must be a device pointer");
-    if (b->timed == 1 && b->planned_groups > 1)
-        return bset_err(b, LVBGS_ERR_ARG, "match-stage timing needs groups = 1");
+    if ((b->timed == 1 || b->timed == 3) && b->planned_groups > 1)
+        return bset_err(b, LVBGS_ERR_ARG, "kernel-stage timing needs groups = 1");
     for (int k = 0; k < ns; ++k) {
         const int s = b->order[(size_t)k];
         View v = b->ctx[s]->v;
@@ -1087,9 +1094,9 @@ extern "C" int lvbgs_batch_apply_device(lvbgs_batch* b, const void* const* d_img
         ++b->ev_used;
     }
     const int G = b->planned_groups;
-    if (b->timed == 1) {
+    if (b->timed == 1 || b->timed == 3) {
         // kernel-stage timing (one group): events between the launches, so the step is issued directly
-        o.timed_stage = 1;
+        o.timed_stage = b->timed;
         batch_step(b->d_views, ns, b->grp[0].prefix, b->grp[0].total, o, b->stream);
         return hipGetLastError() == hipSuccess ? LVBGS_OK : bset_err(b, LVBGS_ERR_HIP, "batch launch");
     }
@@ -1153,7 +1160,8 @@ extern "C" int lvbgs_batch_set_timing(lvbgs_batch* b, const char* stage) {
     if (s.empty()) b->timed = 0;
     else if (s == "match") b->timed = 1;
     else if (s == "step") b->timed = 2;
-    else return bset_err(b, LVBGS_ERR_ARG, "timing stage must be \"match\", \"step\" or \"\"");
+    else if (s == "update") b->timed = 3;
+    else return bset_err(b, LVBGS_ERR_ARG, "timing stage must be \"match\", \"update\", \"step\" or \"\"");
     (void)hipSetDevice(b->device);
     (void)hipStreamSynchronize(b->stream);
     b->ev_used = 0;

@@ -2316,6 +2316,12 @@ __global__ void k_rng_probe(View v, uint32_t frame, uint32_t phase, uint32_t ev,
     for (int i = 0; i < 4; ++i) out[(size_t)p * 4 + i] = d[i];
 }
 
+// sticky device error bits of each stream of a step, gathered for one copy to the host
+__global__ void k_gather_errors(const View* __restrict__ views, int n, uint32_t* __restrict__ out) {
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i < n) out[i] = __hip_atomic_load(&views[i].sc->err_flags, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // ==============================================================================================
 // kernel entry points. Single stream: the View by value, block id remapped per XCD. Batched: every
 // stream of the batch in one launch; a workgroup finds its stream in a per-kernel prefix table of
@@ -2470,7 +2476,9 @@ static void frame_step_t(const View& v, const StepOptions& opt, hipStream_t st) 
     hipLaunchKernelGGL((k_fixpoint_tail<C, ALGO>), dim3(1), dim3(256), 0, st, v, (uint32_t)opt.fixpoint_launches + 1u,
                        (uint32_t)opt.tail_cap);
     if constexpr (ALGO == kSuBSENSE) {
+        if (opt.timed_stage == 3) (void)hipEventRecord(opt.ev_begin, st);
         hipLaunchKernelGGL((k_update<C, ALGO, true>), dim3(g[kGridTile]), dim3(TPB), 0, st, v);
+        if (opt.timed_stage == 3) (void)hipEventRecord(opt.ev_end, st);
         hipLaunchKernelGGL(k_zone_close, dim3(g[kGridZone]), dim3(TPB), 0, st, v);
         hipLaunchKernelGGL(k_zone_link, dim3(g[kGridZone]), dim3(TPB), 0, st, v);
         hipLaunchKernelGGL(k_zone_holes, dim3(g[kGridZone]), dim3(TPB), 0, st, v);
@@ -2481,7 +2489,9 @@ static void frame_step_t(const View& v, const StepOptions& opt, hipStream_t st) 
         hipLaunchKernelGGL((k_refresh<C, ALGO>), dim3(g[kGridSweep]), dim3(TPB), 0, st, v, 1, 0.f, 0, 0u, 0u);
     } else {
         hipLaunchKernelGGL(k_lobster_raw, dim3(g[kGridPix]), dim3(TPB), 0, st, v);
+        if (opt.timed_stage == 3) (void)hipEventRecord(opt.ev_begin, st);
         hipLaunchKernelGGL((k_update<C, ALGO, false>), dim3(g[kGridTile]), dim3(TPB), 0, st, v);
+        if (opt.timed_stage == 3) (void)hipEventRecord(opt.ev_end, st);
         hipLaunchKernelGGL((k_median<false>), dim3(g[kGridPost]), dim3(TPB), 0, st, v);
         if (v.out_mask) hipLaunchKernelGGL(k_copy_mask, dim3(g[kGridPix]), dim3(TPB), 0, st, v);
         hipLaunchKernelGGL((k_copy_last_color<C>), dim3(cdiv((size_t)v.npx * C, TPB)), dim3(TPB), 0, st, v);
@@ -2518,7 +2528,9 @@ void batch_step(const View* d_views, int ns, const uint32_t* const prefix[kNumGr
         hipLaunchKernelGGL(kb_fixpoint, dim3(total[kGridFix]), dim3(TPB), 0, st, args(kGridFix), k);
     hipLaunchKernelGGL(kb_fixpoint_tail, dim3(total[kGridOne]), dim3(256), 0, st, args(kGridOne),
                        (uint32_t)opt.fixpoint_launches + 1u, (uint32_t)opt.tail_cap);
+    if (opt.timed_stage == 3) (void)hipEventRecord(opt.ev_begin, st);
     hipLaunchKernelGGL(kb_update, dim3(total[kGridTile]), dim3(TPB), 0, st, args(kGridTile));
+    if (opt.timed_stage == 3) (void)hipEventRecord(opt.ev_end, st);
     hipLaunchKernelGGL(kb_zone_close, dim3(total[kGridZone]), dim3(TPB), 0, st, args(kGridZone));
     hipLaunchKernelGGL(kb_zone_link, dim3(total[kGridZone]), dim3(TPB), 0, st, args(kGridZone));
     hipLaunchKernelGGL(kb_zone_holes, dim3(total[kGridZone]), dim3(TPB), 0, st, args(kGridZone));
@@ -2567,6 +2579,10 @@ void launch_postprocess_probe(const View& v, hipStream_t st) {
     hipLaunchKernelGGL(k_zone_link, dim3(g[kGridZone]), dim3(TPB), 0, st, v);
     hipLaunchKernelGGL(k_zone_holes, dim3(g[kGridZone]), dim3(TPB), 0, st, v);
     hipLaunchKernelGGL((k_median<true>), dim3(g[kGridPost]), dim3(TPB), 0, st, v);
+}
+
+void launch_gather_errors(const View* d_views, int n, uint32_t* d_out, hipStream_t st) {
+    if (n > 0) hipLaunchKernelGGL(k_gather_errors, dim3(cdiv((size_t)n, 256)), dim3(256), 0, st, d_views, n, d_out);
 }
 
 void launch_rng_probe(const View& v, uint32_t frame, uint32_t phase, uint32_t ev, uint32_t n, uint32_t* out,

@@ -21,7 +21,7 @@ struct StepOptions {
     int cont_blocks = 1024;      // grid of the cooperative phase (grid-stride over unfinished pixels)
     int fix_px_per_block = 0;    // > 0: fix-point grid of a stream = ceil(npx / this), at most fixpoint_blocks
     int cont_px_per_block = 0;   // > 0: cooperative grid of a stream = ceil(npx / this), at most cont_blocks
-    int timed_stage = 0;         // 0: none, 1: the match kernel, 2: the whole step
+    int timed_stage = 0;         // 0: none, 1: the match kernel (phase A), 2: the whole step, 3: the update kernel
     hipEvent_t ev_begin = nullptr, ev_end = nullptr;
 };
 
@@ -54,6 +54,8 @@ void launch_init_desc(const View& v, const uint8_t* img, hipStream_t st);
 void launch_background(const View& v, uint8_t* out_img, uint16_t* out_desc, hipStream_t st);
 void launch_lbsp_abs(const uint8_t* img, int W, int H, int C, uint32_t t, uint16_t* out, hipStream_t st);
 void launch_postprocess_probe(const View& v, hipStream_t st);
+// out[i] = the sticky error bits (Scalars::err_flags) of d_views[i], i < n
+void launch_gather_errors(const View* d_views, int n, uint32_t* d_out, hipStream_t st);
 void launch_rng_probe(const View& v, uint32_t frame, uint32_t phase, uint32_t ev, uint32_t n, uint32_t* out,
                       hipStream_t st);
 

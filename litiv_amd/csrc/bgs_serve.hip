@@ -7,6 +7,13 @@
 // frame of every stream that has one (round-robin start, at most max_batch streams). A stream may
 // have frames in several queued steps: the steps run in order on one HIP stream, so each frame
 // sees the model its predecessor left. Frames arriving while steps run join the next step.
+//
+// Failure isolation: after each step the sticky device error bits of its streams (Scalars::
+// err_flags, e.g. the fix-point iteration cap) are gathered into one pinned copy. At harvest a
+// stream whose bits are set has that frame, its later queued frames and its waiting frames
+// completed with LVBGS_ERR_DEVICE and is closed by the server; its id stays reserved until
+// lvbgs_server_close acknowledges the failure, then the context goes to the pool (re-initialization
+// clears the bits). The other streams of the step are unaffected.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -36,12 +43,20 @@ struct Slot {
     int inflight = 0;  // frames of this stream in queued steps
     bool open = false;
     bool closing = false;
+    bool failed = false;  // a step reported a device error for this stream: closed by the server
     bool ready = false;  // its initialization (queued on the context's stream) has completed
+};
+
+struct StepFrame {
+    int stream;
+    uint64_t tag;
+    int pos;  // the stream's View slot in the step (index into the gathered error bits)
 };
 
 struct QueuedStep {
     hipEvent_t done;
-    std::vector<std::pair<int, uint64_t>> frames;  // (stream, tag)
+    uint32_t* h_err;  // pinned: error bits of the step's View slots, written after the step
+    std::vector<StepFrame> frames;
 };
 
 }  // namespace
@@ -64,6 +79,8 @@ struct lvbgs_server {
     std::vector<bool> stage_live;
     int stage_next = 0;
     std::vector<hipEvent_t> ev_free;
+    std::vector<uint32_t*> err_free;  // pinned error-bit buffers (max_batch words), one per queued step
+    uint32_t* d_err = nullptr;        // device error bits of the last step (max_batch words)
     std::deque<QueuedStep> queued;
     std::vector<lvbgs_done> done;
     int rr = 0;
@@ -106,6 +123,9 @@ static void server_release(lvbgs_server* s) {
     for (auto* c : s->pool) lvbgs_destroy(c);
     if (s->d_views) (void)hipFree(s->d_views);
     if (s->d_prefix) (void)hipFree(s->d_prefix);
+    if (s->d_err) (void)hipFree(s->d_err);
+    for (auto* p : s->err_free) (void)hipHostFree(p);
+    for (auto& q : s->queued) (void)hipHostFree(q.h_err);
     for (auto* p : s->stage) (void)hipHostFree(p);
     for (auto e : s->stage_ev) (void)hipEventDestroy(e);
     for (auto e : s->ev_free) (void)hipEventDestroy(e);
@@ -145,7 +165,8 @@ extern "C" int lvbgs_server_create(int device, const lvbgs_server_options* opts,
     s->groups = L.groups;
     bool ok = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) == hipSuccess &&
               hipMalloc(&s->d_views, sizeof(View) * o.max_batch) == hipSuccess &&
-              hipMalloc(&s->d_prefix, sizeof(uint32_t) * table_words(o.max_batch)) == hipSuccess;
+              hipMalloc(&s->d_prefix, sizeof(uint32_t) * table_words(o.max_batch)) == hipSuccess &&
+              hipMalloc(&s->d_err, sizeof(uint32_t) * (size_t)o.max_batch) == hipSuccess;
     s->gstream[0] = s->stream;
     for (int g = 1; ok && g < s->groups; ++g)
         ok = hipStreamCreateWithFlags(&s->gstream[g], hipStreamNonBlocking) == hipSuccess &&
@@ -163,8 +184,11 @@ extern "C" int lvbgs_server_create(int device, const lvbgs_server_options* opts,
     }
     for (int i = 0; ok && i < o.max_inflight; ++i) {
         hipEvent_t e = nullptr;
-        ok = hipEventCreateWithFlags(&e, hipEventDisableTiming) == hipSuccess;
+        uint32_t* h = nullptr;
+        ok = hipEventCreateWithFlags(&e, hipEventDisableTiming) == hipSuccess &&
+             hipHostMalloc((void**)&h, sizeof(uint32_t) * (size_t)o.max_batch) == hipSuccess;
         if (e) s->ev_free.push_back(e);
+        if (h) s->err_free.push_back(h);
     }
     if (!ok) {
         server_release(s);
@@ -185,6 +209,18 @@ static bool same_model_params(const lvbgs_params& a, const lvbgs_params& b) {
            a.rel_lbsp_threshold == b.rel_lbsp_threshold && a.median_blur_kernel == b.median_blur_kernel;
 }
 
+// a stream whose step reported a device error: its waiting frames complete with the error and the
+// server closes it
+static void fail_stream(lvbgs_server* s, int id) {
+    Slot& sl = s->slots[(size_t)id];
+    if (sl.failed) return;
+    sl.failed = true;
+    sl.closing = true;
+    for (const Request& r : sl.pending) s->done.push_back(lvbgs_done{r.tag, id, LVBGS_ERR_DEVICE});
+    s->st.frames_done += sl.pending.size();
+    sl.pending.clear();
+}
+
 // finished steps -> completions; closing streams whose work is done go to the pool
 static int harvest(lvbgs_server* s, bool wait_one) {
     while (!s->queued.empty()) {
@@ -194,17 +230,20 @@ static int harvest(lvbgs_server* s, bool wait_one) {
         if (e == hipErrorNotReady) break;
         if (e != hipSuccess) return srv_err(s, LVBGS_ERR_HIP, "step failed");
         for (auto& f : q.frames) {
-            s->done.push_back(lvbgs_done{f.second, f.first, LVBGS_OK});
-            Slot& sl = s->slots[(size_t)f.first];
+            Slot& sl = s->slots[(size_t)f.stream];
+            const bool bad = (q.h_err[f.pos] & kErrFixpointCap) != 0 || sl.failed;
+            s->done.push_back(lvbgs_done{f.tag, f.stream, bad ? LVBGS_ERR_DEVICE : LVBGS_OK});
             --sl.inflight;
+            if (bad) fail_stream(s, f.stream);
         }
         s->st.frames_done += q.frames.size();
         s->ev_free.push_back(q.done);
+        s->err_free.push_back(q.h_err);
         s->queued.pop_front();
     }
     for (size_t i = 0; i < s->slots.size(); ++i) {
         Slot& sl = s->slots[i];
-        if (sl.open && sl.closing && sl.pending.empty() && sl.inflight == 0) {
+        if (sl.open && sl.closing && !sl.failed && sl.pending.empty() && sl.inflight == 0) {
             s->pool.push_back(sl.ctx);
             sl = Slot();
             s->free_ids.push_back((int)i);
@@ -246,7 +285,7 @@ static int launch_step(lvbgs_server* s) {
         sl.pending.pop_front();
         ++sl.inflight;
         ++sl.ctx->apply_count;
-        q.frames.emplace_back(members[(size_t)m], req[(size_t)m].tag);
+        q.frames.push_back(StepFrame{members[(size_t)m], req[(size_t)m].tag, 0});
     }
     // groups: largest frames first, dealt round-robin; a group is a contiguous range of View slots
     const int G = std::min(s->groups, n);
@@ -264,6 +303,7 @@ static int launch_step(lvbgs_server* s) {
         for (int j = g; j < n; j += G, ++pos) {
             const int m = by_size[(size_t)j];
             const Request& r = req[(size_t)m];
+            q.frames[(size_t)m].pos = pos;
             lvbgs_ctx* c = s->slots[(size_t)members[(size_t)m]].ctx;
             View v = c->v;
             v.frame = r.img;
@@ -305,10 +345,17 @@ static int launch_step(lvbgs_server* s) {
         if (hipEventRecord(s->ev_join[g], s->gstream[g]) != hipSuccess ||
             hipStreamWaitEvent(s->stream, s->ev_join[g], 0) != hipSuccess)
             return srv_err(s, LVBGS_ERR_HIP, "join");
+    // the step's error bits, copied out before its completion event
     q.done = s->ev_free.back();
     s->ev_free.pop_back();
-    if (hipEventRecord(q.done, s->stream) != hipSuccess || hipGetLastError() != hipSuccess) {
+    q.h_err = s->err_free.back();
+    s->err_free.pop_back();
+    launch_gather_errors(s->d_views, n, s->d_err, s->stream);
+    if (hipMemcpyAsync(q.h_err, s->d_err, sizeof(uint32_t) * (size_t)n, hipMemcpyDeviceToHost, s->stream) !=
+            hipSuccess ||
+        hipEventRecord(q.done, s->stream) != hipSuccess || hipGetLastError() != hipSuccess) {
         s->ev_free.push_back(q.done);
+        s->err_free.push_back(q.h_err);
         return srv_err(s, LVBGS_ERR_HIP, "step launch");
     }
     s->queued.push_back(std::move(q));
@@ -425,6 +472,11 @@ extern "C" int lvbgs_server_open(lvbgs_server* s, const lvbgs_params* params, co
 }
 
 extern "C" int lvbgs_server_close(lvbgs_server* s, int stream) {
+    if (s && stream >= 0 && stream < (int)s->slots.size() && s->slots[(size_t)stream].failed) {
+        // closed by the server after a device error: the call releases the id
+        s->slots[(size_t)stream].failed = false;
+        return harvest(s, false);
+    }
     if (!s || stream < 0 || stream >= (int)s->slots.size() || !s->slots[(size_t)stream].open ||
         s->slots[(size_t)stream].closing)
         return s ? srv_err(s, LVBGS_ERR_ARG, "unknown stream") : LVBGS_ERR_ARG;
@@ -434,6 +486,8 @@ extern "C" int lvbgs_server_close(lvbgs_server* s, int stream) {
 
 extern "C" int lvbgs_server_submit(lvbgs_server* s, int stream, const void* d_img, void* d_mask, double lr,
                                    uint64_t tag) {
+    if (s && stream >= 0 && stream < (int)s->slots.size() && s->slots[(size_t)stream].failed)
+        return srv_err(s, LVBGS_ERR_DEVICE, "stream closed after a device error");
     if (!s || stream < 0 || stream >= (int)s->slots.size() || !s->slots[(size_t)stream].open ||
         s->slots[(size_t)stream].closing || !d_img)
         return s ? srv_err(s, LVBGS_ERR_ARG, "unknown or closing stream, or null frame") : LVBGS_ERR_ARG;
@@ -445,6 +499,12 @@ extern "C" int lvbgs_server_submit(lvbgs_server* s, int stream, const void* d_im
 extern "C" lvbgs_ctx* lvbgs_server_context(lvbgs_server* s, int stream) {
     if (!s || stream < 0 || stream >= (int)s->slots.size() || !s->slots[(size_t)stream].open) return nullptr;
     return s->slots[(size_t)stream].ctx;
+}
+
+extern "C" int lvbgs_server_debug_fixpoint_cap(lvbgs_server* s, int tail_cap) {
+    if (!s || tail_cap < 0) return LVBGS_ERR_ARG;
+    s->opt.tail_cap = tail_cap;
+    return LVBGS_OK;
 }
 
 extern "C" int lvbgs_server_get_stats(lvbgs_server* s, lvbgs_server_stats* out) {

@@ -125,3 +125,72 @@ def test_batch_one_stream_per_class_full_size():
     for i in range(len(ids)):
         _check_state(gpus[i], orcs[i], SUB_MAPS + SAMPLES, "stream %d" % ids[i])
     batch.close()
+
+
+def _oracle_stream(sid, n_frames, keep_samples):
+    """Stream `sid` of the 64-stream mix through the oracle: per-frame masks and counters, end state."""
+    seq = synth.Sequence(synth.preset(synth.BATCH64, sid))
+    o = Oracle(SUBSENSE, stream_id=sid)
+    o.initialize(seq.frame(0), None)
+    masks, counters = [], []
+    for t in range(n_frames):
+        masks.append(o.apply(seq.frame(t), None))
+        counters.append(o.counters())
+    names = SUB_MAPS + (SAMPLES if keep_samples else [])
+    return masks, counters, {k: o.state(k) for k in names}, o.scalars()
+
+
+def test_batch64_all_streams_with_the_tuned_options():
+    """All 64 streams of the 64-stream mix in one batch with the options the online tuner settles on
+    (three concurrent groups, graph replay, fetch depth 4, two full-grid fix-point launches,
+    sequential limit 12 and 8 for the 1920x1080 class): every stream against its own oracle (run in
+    host threads), masks and counters on every frame, maps and scalars at the end (sample planes for
+    one stream per class)."""
+    from concurrent.futures import ThreadPoolExecutor
+    torch = _torch()
+    n_frames = 4
+    sids = list(range(64))
+    keep = {0, 16, 32, 48}
+    pool = ThreadPoolExecutor(max_workers=16)
+    futs = {sid: pool.submit(_oracle_stream, sid, n_frames, sid in keep) for sid in sids}
+    seqs = [synth.Sequence(synth.preset(synth.BATCH64, sid)) for sid in sids]
+    bufs = [torch.empty(n_frames * s.frame_bytes, dtype=torch.uint8, device="cuda") for s in seqs]
+    masks = [torch.empty((s.cfg.height, s.cfg.width), dtype=torch.uint8, device="cuda") for s in seqs]
+    torch.cuda.synchronize()
+    for s, b in zip(seqs, bufs):
+        s.frames_device(0, n_frames, b.data_ptr())
+    torch.cuda.synchronize()
+    gpus = [BackgroundSubtractorSuBSENSE(stream_id=sid) for sid in sids]
+    for g, s in zip(gpus, seqs):
+        g.initialize(s.frame(0))
+    batch = Batch(gpus, use_graph=1, groups=3, seq_limit=12, fetch_depth=4, fixpoint_launches=2,
+                  cont_px_per_block=2048)
+    for i in range(48, 64):
+        batch.set_stream_options(i, seq_limit=8)
+    got_masks = {sid: [] for sid in sids}
+    got_counters = {sid: [] for sid in sids}
+    for t in range(n_frames):
+        print("batch64 frame %d" % t, flush=True)
+        batch.apply_device([b.data_ptr() + t * s.frame_bytes for b, s in zip(bufs, seqs)],
+                           [m.data_ptr() for m in masks])
+        batch.synchronize()
+        for sid in sids:
+            got_masks[sid].append(masks[sid].cpu().numpy())
+            st = gpus[sid].stats()
+            got_counters[sid].append((st["examined"], st["writes"]))
+    batch.close()
+    for sid in sids:
+        om, oc, ostate, osc = futs[sid].result()
+        for t in range(n_frames):
+            assert np.array_equal(got_masks[sid][t], om[t]), "stream %d frame %d mask: %s" % (
+                sid, t, _diff(got_masks[sid][t], om[t]))
+            assert got_counters[sid][t] == oc[t], "stream %d frame %d counters" % (sid, t)
+        for name, so in ostate.items():
+            sg = gpus[sid].state(name)
+            assert np.array_equal(sg, so), "stream %d state %s: %s" % (sid, name, _diff(sg, so))
+        sc = gpus[sid].scalars()
+        for k in ("frame_idx", "frames_since_reset", "cooldown", "auto_reset", "lower_cap", "upper_cap",
+                  "refresh_seq", "median_k", "use3x3"):
+            assert sc[k] == osc[k], "stream %d scalar %s" % (sid, k)
+        futs[sid] = None
+    pool.shutdown()

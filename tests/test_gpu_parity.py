@@ -175,3 +175,53 @@ def test_lobster_refresh_and_background():
             orc.refresh_model(0.5)
     assert np.array_equal(gpu.getBackgroundImage(), orc.background_image())
     assert np.array_equal(gpu.getBackgroundDescriptorsImage(), orc.background_descriptors())
+
+
+def _continue(gpu, orc, frames, names=SUB_STATE):
+    for t, f in enumerate(frames):
+        mg, mo = gpu.apply(f), orc.apply(f)
+        assert np.array_equal(mg, mo), "frame %d mask: %s" % (t, _describe_diff(mg, mo))
+    for name in names:
+        sg, so = gpu.state(name), orc.state(name)
+        assert np.array_equal(sg, so), "state %s: %s" % (name, _describe_diff(sg, so))
+    sg, so = gpu.scalars(), orc.scalars()
+    for k in ("frame_idx", "frames_since_reset", "cooldown", "auto_reset", "lr_scaling", "use3x3", "median_k",
+              "n_roi", "lower_cap", "upper_cap", "refresh_seq"):
+        assert sg[k] == so[k], "scalar %s gpu %r oracle %r" % (k, sg[k], so[k])
+
+
+def test_reinitialize_without_roi_recounts_the_stored_roi():
+    # A left band of 158 of 320 columns counts 160 columns with its dilated ring: half the frame, so
+    # learning-rate scaling is on (BackgroundSubtractorSuBSENSE.cpp:112). Re-initializing without a
+    # ROI reuses the stored, border-trimmed ROI (BackgroundSubtractionUtils.cpp:87-88) and recounts it
+    # (:97): 236 x 158 pixels, under half, so the second initialize turns scaling off.
+    frames = seq(n=12)
+    roi = np.zeros((240, 320), np.uint8)
+    roi[:, :158] = 255
+    gpu, orc = _pair(SUBSENSE)
+    gpu.initialize(frames[0], roi)
+    orc.initialize(frames[0], roi)
+    assert gpu.scalars()["lr_scaling"] == 1 == orc.scalars()["lr_scaling"]
+    _continue(gpu, orc, frames[:6])
+    gpu.initialize(frames[6])
+    orc.initialize(frames[6])
+    assert gpu.scalars()["lr_scaling"] == 0 == orc.scalars()["lr_scaling"]
+    assert gpu.scalars()["n_roi"] == 236 * 158
+    _continue(gpu, orc, frames[6:])
+
+
+def test_set_roi_on_an_initialized_context():
+    # setROI after initialize re-initializes from the current background image with the new ROI
+    # (BackgroundSubtractionUtils.cpp:38-48); the oracle does the same two calls
+    frames = seq(n=16, width=160, height=120)
+    gpu, orc = _pair(SUBSENSE)
+    gpu.initialize(frames[0])
+    orc.initialize(frames[0])
+    _continue(gpu, orc, frames[:6])
+    roi = np.zeros((120, 160), np.uint8)
+    roi[20:100, 30:140] = 255
+    gpu.setROI(roi)
+    bg = orc.background_image()
+    orc.initialize(bg, gpu.validateROI(roi))
+    assert np.array_equal(gpu.getROICopy(), orc.state("roi"))
+    _continue(gpu, orc, frames[6:])

@@ -138,3 +138,149 @@ def test_server_rejects_bad_calls():
     assert Server(hbm_budget=1024).open(f.data_ptr(), 64, 48, 3) is None
     srv.drain()
     srv.close_server()
+
+
+def _serve_oracle(kind, k, n, params):
+    seq = synth.Sequence(synth.preset(kind, k))
+    o = Oracle(SUBSENSE, **params)
+    o.initialize(seq.frame(0))
+    return [o.apply(seq.frame(t)) for t in range(n)]
+
+
+def test_serving_full_size_resolutions_with_admission_churn():
+    """Streams of the serving workload's four resolutions (320x240, 640x480, 1280x720 twice,
+    1920x1080 RGB) at full size. The budget holds four of the five models: the fifth waits, is
+    admitted when the first stream closes (evicting its pooled context), and every frame of every
+    stream is checked against that stream's oracle (run in host threads)."""
+    from concurrent.futures import ThreadPoolExecutor
+    torch = _torch()
+    want = [(320, 240), (640, 480), (1280, 720), (1280, 720), (1920, 1080)]
+    picks, k = [], 0
+    while len(picks) < len(want):
+        cfg = synth.preset(synth.SERVE, k)
+        if (cfg.width, cfg.height) == want[len(picks)]:
+            picks.append(k)
+        k += 1
+    n = 8
+    params = [{"stream_id": 2000 + k} for k in picks]
+    pool = ThreadPoolExecutor(max_workers=len(picks))
+    futs = [pool.submit(_serve_oracle, synth.SERVE, k, n, p) for k, p in zip(picks, params)]
+    seqs = [synth.Sequence(synth.preset(synth.SERVE, k)) for k in picks]
+    bufs = [torch.empty(n * s.frame_bytes, dtype=torch.uint8, device="cuda") for s in seqs]
+    masks = [torch.empty((n, s.cfg.height, s.cfg.width), dtype=torch.uint8, device="cuda") for s in seqs]
+    torch.cuda.synchronize()
+    for s, b in zip(seqs, bufs):
+        s.frames_device(0, n, b.data_ptr())
+    torch.cuda.synchronize()
+    sizes = [Server.model_bytes(s.cfg.width, s.cfg.height, 3) for s in seqs]
+    # order of opening: the 1080p stream first, then 320x240, 640x480, 1280x720; the second 1280x720
+    # stream does not fit until the 320x240 one has left and its pooled context is evicted
+    order = [4, 0, 1, 2, 3]
+    budget = sizes[4] + sizes[1] + sizes[2] + sizes[3] + sizes[0] // 2
+    srv = Server(hbm_budget=budget, max_inflight=2, groups=2, seq_limit=8)
+    ids = {}
+    for i in order[:4]:
+        ids[i] = srv.open(bufs[i].data_ptr(), seqs[i].cfg.width, seqs[i].cfg.height, 3, **params[i])
+        assert ids[i] is not None
+    assert srv.open(bufs[3].data_ptr(), seqs[3].cfg.width, seqs[3].cfg.height, 3, **params[3]) is None
+    nxt = {i: 0 for i in range(len(picks))}
+    status = {}
+
+    def submit(i, count):
+        for _ in range(count):
+            t = nxt[i]
+            if t >= n:
+                return
+            srv.submit(ids[i], bufs[i].data_ptr() + t * seqs[i].frame_bytes, masks[i][t].data_ptr(), 0.0,
+                       tag=i * 1000 + t)
+            nxt[i] += 1
+
+    def collect():
+        srv.pump()
+        for tag, sid, st in srv.poll():
+            status[int(tag)] = int(st)
+
+    rng = np.random.default_rng(3)
+    while nxt[0] < n:
+        for i in order[:4]:
+            submit(i, int(rng.integers(1, 3)))
+        collect()
+    srv.close(ids[0])
+    srv.drain()
+    collect()
+    ids[3] = srv.open(bufs[3].data_ptr(), seqs[3].cfg.width, seqs[3].cfg.height, 3, **params[3])
+    assert ids[3] is not None
+    while any(nxt[i] < n for i in (1, 2, 3, 4)):
+        for i in (1, 2, 3, 4):
+            submit(i, int(rng.integers(0, 3)))
+        collect()
+    srv.drain()
+    collect()
+    assert len(status) == n * len(picks) and all(v == 0 for v in status.values())
+    torch.cuda.synchronize()
+    for i in range(len(picks)):
+        exp = futs[i].result()
+        got = masks[i].cpu().numpy()
+        for t in range(n):
+            assert np.array_equal(got[t], exp[t]), "%dx%d stream frame %d: %d px differ" % (
+                seqs[i].cfg.width, seqs[i].cfg.height, t, int((got[t] != exp[t]).sum()))
+    pool.shutdown()
+    srv.close_server()
+
+
+def test_device_error_is_reported_for_one_stream_only():
+    """The fix-point cap set to 0 from the host (no full-grid fix-point launches): a stream whose raster-
+    order fix point has work stops with LVBGS_ERR_DEVICE (every kernel exits normally); a static
+    stream (no decision ever changes, no fix-point work) in the same steps keeps its exact results.
+    The failed stream's later frames complete with the same status, further submits are refused, and
+    after close (which acknowledges the failure) its pooled context serves a new stream exactly."""
+    from litiv_amd.bgs import ERR_DEVICE, LvbgsError
+    torch = _torch()
+    n = 10
+    moving = Stream(torch, 0, 96, 72, 3, n)
+    still = Stream(torch, 1, 80, 64, 3, n)
+    f0 = still.frames[0]
+    still.frames = [f0] * n
+    still.d = [still.d[0]] * n
+    o = Oracle(SUBSENSE, **still.params)
+    o.initialize(f0)
+    still.expected = [o.apply(f0) for _ in range(n)]
+    torch.cuda.synchronize()
+    srv = Server(hbm_budget=1 << 30, max_inflight=2, fixpoint_launches=0)
+    srv.debug_fixpoint_cap(0)
+    sm, ss = _open(srv, moving), _open(srv, still)
+    for t in range(n):
+        srv.submit(sm, moving.d[t].data_ptr(), moving.m[t].data_ptr(), 0.0, tag=t)
+        srv.submit(ss, still.d[t].data_ptr(), still.m[t].data_ptr(), 0.0, tag=100 + t)
+    srv.drain()
+    done = srv.poll()
+    st_m = {int(tag): int(s) for tag, sid, s in done if sid == sm}
+    st_s = {int(tag) - 100: int(s) for tag, sid, s in done if sid == ss}
+    assert sorted(st_m) == list(range(n)) and sorted(st_s) == list(range(n))
+    assert all(v == 0 for v in st_s.values())
+    bad = [t for t in range(n) if st_m[t] != 0]
+    assert bad, "the moving stream never needed the fix point"
+    assert all(st_m[t] == ERR_DEVICE for t in bad) and bad == list(range(bad[0], n))
+    for t in range(bad[0]):
+        assert np.array_equal(moving.m[t].cpu().numpy(), moving.expected[t]), t
+    for t in range(n):
+        assert np.array_equal(still.m[t].cpu().numpy(), still.expected[t]), t
+    with pytest.raises(LvbgsError) as e:
+        srv.submit(sm, moving.d[0].data_ptr(), moving.m[0].data_ptr())
+    assert e.value.code == ERR_DEVICE
+    srv.close(sm)
+    srv.drain()
+    assert srv.stats()["pooled_contexts"] == 1
+    # default cap again: a new stream of the same geometry reuses the pooled context, flags cleared
+    srv.debug_fixpoint_cap(1 << 16)
+    again = Stream(torch, 0, 96, 72, 3, n)
+    torch.cuda.synchronize()
+    sa = _open(srv, again)
+    assert srv.stats()["pooled_contexts"] == 0
+    for t in range(n):
+        srv.submit(sa, again.d[t].data_ptr(), again.m[t].data_ptr(), 0.0, tag=200 + t)
+    srv.drain()
+    assert all(int(s) == 0 for _, _, s in srv.poll())
+    for t in range(n):
+        assert np.array_equal(again.m[t].cpu().numpy(), again.expected[t]), t
+    srv.close_server()
