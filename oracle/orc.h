@@ -14,6 +14,7 @@
  *   modules/features2d/include/litiv/features2d/LBSP.hpp:143-319
  *   modules/utils/include/litiv/utils/math.hpp:199-234,628-735
  *   modules/utils/include/litiv/utils/opencv.hpp:859-966
+ *   modules/datasets/src/metrics.cpp:21-50 (orc_eval.c)
  *
  * Deliberate, documented differences from the reference binary:
  *   - every libc rand() call is replaced by a counter-based Philox4x32-10 draw
@@ -94,6 +95,10 @@ void     orc_morph(const uint8_t* in, uint8_t* out, int w, int h, int radius, in
 void     orc_median_binary(const uint8_t* in, uint8_t* out, int w, int h, int k);
 void     orc_flood_holes(const uint8_t* in, uint8_t* holes, int w, int h);
 void     orc_resize_area(const uint8_t* in, int w, int h, int c, uint8_t* out, int dw, int dh);
+/* binary classification counts of a mask against ground truth (orc_eval.c), accumulated into
+ * out = {TP, TN, FP, FN, SE, DC}; gt and roi may be NULL */
+void     orc_binclassif(const uint8_t* classif, const uint8_t* gt, const uint8_t* roi, int w, int h, size_t stride,
+                        uint64_t out[6]);
 
 #ifdef __cplusplus
 }

@@ -73,6 +73,8 @@ def lib():
         L.orc_median_binary.argtypes = [u8p, u8p, C.c_int, C.c_int, C.c_int]
         L.orc_flood_holes.argtypes = [u8p, u8p, C.c_int, C.c_int]
         L.orc_resize_area.argtypes = [u8p, C.c_int, C.c_int, C.c_int, u8p, C.c_int, C.c_int]
+        L.orc_binclassif.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_size_t,
+                                     C.POINTER(C.c_uint64)]
         _lib = L
     return _lib
 
@@ -193,3 +195,15 @@ class Oracle:
         a, b = C.c_uint64(), C.c_uint64()
         lib().orc_counters(self.h, C.byref(a), C.byref(b))
         return a.value, b.value
+
+
+def binclassif(classif, gt=None, roi=None):
+    """The oracle's BinClassif counts (TP, TN, FP, FN, SE, DC) of 2-D uint8 masks."""
+    classif = np.ascontiguousarray(classif, dtype=np.uint8)
+    h, w = classif.shape
+    out = (C.c_uint64 * 6)()
+    g = None if gt is None else np.ascontiguousarray(gt, dtype=np.uint8)
+    r = None if roi is None else np.ascontiguousarray(roi, dtype=np.uint8)
+    lib().orc_binclassif(classif.ctypes.data, None if g is None else g.ctypes.data,
+                         None if r is None else r.ctypes.data, w, h, w, out)
+    return tuple(int(x) for x in out)

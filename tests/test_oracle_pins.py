@@ -221,3 +221,29 @@ def test_resize_area_integer_factor_is_rounded_block_mean():
     # sums / 64 are exact in float32, so the result is the half-to-even rounded block mean
     mean = img.reshape(6, 8, 8, 8, 3).astype(np.int64).sum(axis=(1, 3)) / 64.0
     assert np.array_equal(out, np.rint(mean).astype(np.uint8))
+
+
+# ---- BinClassif (modules/datasets/src/metrics.cpp:21-50) ------------------------------------------
+# One pixel at a time, the category each (result, ground truth, ROI) triple takes by the reference's
+# tests: GT 85 / 170 or ROI 0 -> don't care; else result 255 -> TP (GT 255) / FP (any other GT);
+# result != 255 -> FN (GT 255) / TN; result 255 on GT 50 -> also SE.
+def _expected_category(res, gt, roi):
+    if gt in (85, 170) or roi == 0:
+        return {"dc"}
+    cat = {("tp" if gt == 255 else "fp") if res == 255 else ("fn" if gt == 255 else "tn")}
+    if gt == 50 and res == 255:
+        cat.add("se")
+    return cat
+
+
+def test_binclassif_known_answers():
+    names = ("tp", "tn", "fp", "fn", "se", "dc")
+    for res in (0, 7, 255):
+        for gt in (0, 3, 50, 85, 170, 255):
+            for roi in (None, 0, 1, 255):
+                got = oracle_lib.binclassif(np.full((1, 1), res, np.uint8), np.full((1, 1), gt, np.uint8),
+                                            None if roi is None else np.full((1, 1), roi, np.uint8))
+                assert {n for n, v in zip(names, got) if v} == _expected_category(res, gt, roi), (res, gt, roi)
+                assert sum(got) - got[4] == 1
+    # no ground truth: every pixel is a don't-care (metrics.cpp:26-29)
+    assert oracle_lib.binclassif(np.zeros((3, 5), np.uint8)) == (0, 0, 0, 0, 0, 15)
