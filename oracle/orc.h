@@ -24,8 +24,10 @@
  *
  * Pins: tests/test_oracle_pins.py checks each primitive against an outside reference (Random123
  * Philox vectors, the reference's LBSP golden data, the reference's sampling table, scipy's
- * morphology/median/hole filling, closed forms); tests/test_oracle_behaviour.py checks the whole
- * apply() on scenes with known answers.
+ * morphology/median/hole filling, closed forms); tests/test_oracle_apply_pins.py checks the whole
+ * apply(), refreshModel and initialize against closed forms derived by hand from the reference's
+ * lines (threshold flip points, feedback trajectories, controller reset frame and caps, scaling
+ * table); tests/test_oracle_behaviour.py checks behaviour on scenes with known answers.
  * Parity unpinned: the fp32 rounding of OpenCV's addWeighted / accumulateWeighted (restated here as
  * fixed unfused formulas), orc_resize_area for non-integer factors.
  */
