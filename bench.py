@@ -7,14 +7,18 @@ on the GPU before the timed region (identical to the host generator's frames).
 
 Default workload: 64 SuBSENSE streams (16x 320x240 RGB, 16x 640x480 RGB, 16x 640x512 grey,
 16x 1920x1080 RGB; independent generator seeds; 600 frames each) stepped as one batch (one launch
-per kernel for all streams, replayed from a hipGraph). The same 64 streams are also run serially
-through the plain one-stream drop-in (apply + synchronize per stream) over the same frames, and
-the line carries both rates and their ratio.
+per kernel for all streams, replayed from a hipGraph). The launch options come from a tuning pass
+of the online autotuner over the same streams' first frames, run before and apart from the timed
+region; the models are then re-initialized and the timed steps replay the frozen choices. The same
+64 streams are also run serially through the plain one-stream drop-in (apply + synchronize per
+stream) over the same frames, and the line carries both rates and their ratio.
 
-Extra fields: roofline of the dominant kernel (the speculative match; algorithmic bytes from the
-device's exact examined-sample counters), the CPU baseline (the sequential oracle on a bounded
-sample of the same streams, one stream per host thread), per-step GPU latency p50/p99, fix-point
-statistics. Plain single-stream workloads (1080p, QVGA, 4K, LOBSTER thermal) are selectable.
+Extra fields: roofline of the dominant kernel (the longer of the match's phase A and the model
+update, by HIP-event time; algorithmic bytes from the device's exact examined-sample and
+sample-write counters) with both kernels' figures, the CPU baseline (the sequential oracle on a
+bounded sample of the same streams, one stream per host thread), per-step GPU latency p50/p99,
+fix-point statistics, serving runs. Plain single-stream workloads (1080p, QVGA, 4K, LOBSTER
+thermal) are selectable.
 """
 import argparse
 import json
@@ -85,6 +89,30 @@ def match_bytes(C, n_roi, examined):
     return n_roi * (3 * C + 15) + 3 * C * examined
 
 
+def update_bytes(C, n_roi, writes):
+    """Algorithmic bytes of the update kernel over a stream (SuBSENSE, feedback maps fused in): per ROI
+    pixel it reads roi (1), match result (4), decision (1), record (4), frame (C), intra descriptors
+    (2C), last colour and descriptor (3C), eight f32 maps (32: MLD, D_min LT/ST, raw LT/ST, T, V, R),
+    last_fg, blinks, last raw and last raw blink (4), and writes the eight maps (32), unstable (1),
+    last colour and descriptor (3C), blinks, last raw blink, last raw and raw (4): 79 + 9C bytes;
+    plus 3C bytes (colour + descriptor) per sample write."""
+    return n_roi * (79 + 9 * C) + 3 * C * writes
+
+
+def update_bytes_lobster(C, n_roi, writes):
+    """Algorithmic bytes of LOBSTER's update kernel over a stream: per ROI pixel the record (4), the
+    decision (1), the frame (C) and intra descriptors (2C); plus 3C bytes per sample write."""
+    return n_roi * (5 + 3 * C) + 3 * C * writes
+
+
+def kernel_figures(name, avg_s, bytes_launch):
+    """Roofline figures of one kernel: algorithmic bytes per launch over its mean launch time."""
+    achieved = bytes_launch / avg_s / 1e9
+    return {"bound": "hbm", "kernel": name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "bytes_per_launch": int(bytes_launch),
+            "avg_launch_us": round(avg_s * 1e6, 2)}
+
+
 def step_bytes(C, n_roi, examined, writes):
     """Algorithmic bytes of a whole frame step (per-pixel figure 114 + 9(k+w) for RGB, 100 + 3(k+w) grey)."""
     return n_roi * (114 if C == 3 else 100) + 3 * C * (examined + writes)
@@ -144,14 +172,44 @@ def _oracle_rate(algo, classes, lr, budget_s):
     return total_px / step_s / 1e6, descr
 
 
+def host_cpu():
+    """CPU model name (from /proc/cpuinfo), logical CPUs of the host, CPUs this process may use."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    usable = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    return model or platform.machine(), os.cpu_count(), usable
+
+
+def oracle_flags():
+    """The C compiler flags the oracle library was built with (oracle/Makefile)."""
+    try:
+        with open(os.path.join(ROOT, "oracle", "Makefile")) as f:
+            for line in f:
+                if line.startswith("CFLAGS"):
+                    return line.split("=", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(algo, classes, lr, budget_s):
-    cpu = platform.processor() or platform.machine()
+    model, nproc, usable = host_cpu()
     threads = max(len(streams) for _, streams in classes)
     v, descr = _oracle_rate(algo, classes, lr, budget_s)
     return {"value": round(v, 4), "unit": "Mpix/s", "cores": threads, "kind": "port",
-            "sample": "sequential CPU oracle, one stream per thread (%d thread(s), %s), on the first frames of "
+            "threads": threads, "host_cpu": model, "host_nproc": nproc, "usable_cpus": usable,
+            "compiler": "gcc " + (oracle_flags() or "?"),
+            "sample": "sequential CPU oracle (oracle/, the C restatement of the reference's apply; gcc %s), one "
+                      "stream per thread (%d thread(s) on %s, %s logical CPUs, %s usable), on the first frames of "
                       "streams of each resolution class; per-class rates weighted by the workload's pixel mix: %s"
-                      % (threads, cpu, "; ".join(descr))}
+                      % (oracle_flags(), threads, model, nproc, usable, "; ".join(descr))}
 
 
 # ------------------------------------------------------------------------------------------------
@@ -163,7 +221,8 @@ def run_batch(args, preset, n_streams, ref_frames):
     from litiv_amd.autotune import OnlineTuner
 
     steps = args.steps if args.steps is not None else ref_frames - args.warmup
-    total = args.warmup + steps
+    tune_steps = 0 if args.no_tune else args.tune_steps
+    total = max(args.warmup + steps, tune_steps)
     dev = torch.device("cuda:0")
     cfgs = [synth.preset(preset, s) for s in range(n_streams)]
     seqs = []
@@ -215,60 +274,84 @@ def run_batch(args, preset, n_streams, ref_frames):
         plain = px_step * steps / wall_plain / 1e6
         log("plain serial: %.3f ms/step, %.1f Mpix/s" % (wall_plain / steps * 1e3, plain))
 
-    # (2) the batch, stepped through the online tuner: trial rounds (shared options, or one resolution
-    # class's launch shape) every 32 steps, fewer once the choices settle; warm-up and timed steps
-    # alike (options never change results)
+    # (2) tuning pass, outside the timed region: the online tuner steps the batch over the first
+    # tune_steps frames with a trial round every tune_period steps (a round times each candidate of the
+    # shared options, or a resolution class's launch shape, on live frames; options never change
+    # results). Its choices are then frozen.
+    tuner, tune_info = None, None
+    if tune_steps:
+        init_all()
+        batch = Batch(subs)
+        tuner = OnlineTuner(batch, period=args.tune_period, max_period=args.tune_period, log=log)
+        t0 = time.perf_counter()
+        for t in range(tune_steps):
+            tuner.apply(*ptrs(t))
+        batch.synchronize()
+        log("tuning pass: %d steps, %d rounds, %.1f s" % (tune_steps, tuner.rounds, time.perf_counter() - t0))
+        tuned = batch.options()
+        tune_info = {"pass_steps": tune_steps, "rounds": tuner.rounds, "period": args.tune_period,
+                     "shared_candidates": tuner.shared,
+                     "shared_median_step_ms": [round(m, 4) if m else m for m in tuner.shared_medians()],
+                     "per_resolution": tuner.class_choices()}
+        batch.close()
+
+    # (3) the timed run: models re-initialized, frozen options, the step captured once as a hipGraph
+    # (per group) during the warm-up and replayed
     init_all()
     batch = Batch(subs)
-    tuner = None if args.no_tune else OnlineTuner(batch, log=log)
-    step = tuner.apply if tuner else batch.apply_device
+    if tuner is not None:
+        tuner.apply_choices(batch)
+        batch.set_options(use_graph=1)
+    tuned = batch.options()
     for t in range(args.warmup):
-        step(*ptrs(t))
+        batch.apply_device(*ptrs(t))
     batch.synchronize()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for k in range(steps):
-        step(*ptrs(args.warmup + k))
+        batch.apply_device(*ptrs(args.warmup + k))
     batch.synchronize()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     mpix = px_step * steps / wall / 1e6
-    tuned = batch.options()
     log("batched: %.3f ms/step, %.1f Mpix/s (options %s)" % (wall / steps * 1e3, mpix, tuned))
-    tune_info = None if tuner is None else {"rounds": tuner.rounds, "period": tuner.period,
-                                            "shared_candidates": tuner.shared,
-                                            "shared_median_step_ms": [round(m, 4) if m else m
-                                                                      for m in tuner.shared_medians()],
-                                            "per_resolution": tuner.class_choices()}
     batch.close()
 
-    # (3) measurement passes over the same frames (re-initialized): match-kernel time with HIP events
-    # and exact examined-sample counters per stream, then whole-step latency
+    # (4) measurement passes over the same frames (re-initialized, frozen options): HIP-event time of
+    # the match's phase A and of the update kernel (one group, so one launch of each covers every
+    # stream) with the exact per-stream counters behind their algorithmic bytes, then whole-step
+    # latency
     n_meas = min(steps, args.meas_steps)
-    log("measurement pass: match-kernel timing and counters over %d steps" % n_meas)
-    init_all()
-    batch = Batch(subs, **tuned)
-    if tuner is not None:
-        tuner.apply_choices(batch)
-    batch.set_options(groups=1)  # the roofline kernel: one match launch over every stream
-    for t in range(args.warmup):
-        batch.apply_device(*ptrs(t))
-    batch.set_timing("match")
-    match_ms, mbytes, sbytes = [], 0.0, 0.0
-    iters, work = [], np.zeros(8)
-    for k in range(n_meas):
-        batch.apply_device(*ptrs(args.warmup + k))
-        ms, _ = batch.take_timing()
-        match_ms.append(ms)
-        for i, b in enumerate(subs):
-            st = b.stats()
-            C = st["channels"]
-            mbytes += match_bytes(C, st["n_roi"], st["examined_speculative"])
-            sbytes += step_bytes(C, st["n_roi"], st["examined"], st["writes"])
-            iters.append(st["fixpoint_iters"])
-            work += np.array(st["fixpoint_work"], dtype=np.float64)
-    batch.set_timing("")
-    batch.close()
+    kern = {}
+    sbytes, iters, work = 0.0, [], np.zeros(8)
+    for stage in ("match", "update"):
+        log("measurement pass: %s-kernel timing and counters over %d steps" % (stage, n_meas))
+        init_all()
+        batch = Batch(subs, **tuned)
+        if tuner is not None:
+            tuner.apply_choices(batch)
+        batch.set_options(groups=1)
+        for t in range(args.warmup):
+            batch.apply_device(*ptrs(t))
+        batch.set_timing(stage)
+        ms_list, nbytes = [], 0.0
+        for k in range(n_meas):
+            batch.apply_device(*ptrs(args.warmup + k))
+            ms, _ = batch.take_timing()
+            ms_list.append(ms)
+            for i, b in enumerate(subs):
+                st = b.stats()
+                C = st["channels"]
+                if stage == "match":
+                    nbytes += match_bytes(C, st["n_roi"], st["examined_speculative"])
+                    sbytes += step_bytes(C, st["n_roi"], st["examined"], st["writes"])
+                    iters.append(st["fixpoint_iters"])
+                    work += np.array(st["fixpoint_work"], dtype=np.float64)
+                else:
+                    nbytes += update_bytes(C, st["n_roi"], st["writes"])
+        batch.set_timing("")
+        batch.close()
+        kern[stage] = (float(np.mean(ms_list)) / 1e3, nbytes / n_meas)
     log("measurement pass: step latency over %d steps" % n_meas)
     init_all()
     batch = Batch(subs, **tuned)
@@ -285,9 +368,12 @@ def run_batch(args, preset, n_streams, ref_frames):
     batch.set_timing("")
     batch.close()
     log("step latency p50 %.3f ms, p99 %.3f ms" % (np.percentile(step_ms, 50), np.percentile(step_ms, 99)))
-    avg_match_s = float(np.mean(match_ms)) / 1e3
-    bytes_launch = mbytes / n_meas
-    achieved = bytes_launch / avg_match_s / 1e9
+    kernels = {
+        "match": kernel_figures("kb_match (phase A of the ordered sample match, all streams)", *kern["match"]),
+        "update": kernel_figures("kb_update (model update, diffusion and feedback maps, all streams)",
+                                 *kern["update"]),
+    }
+    dominant = max(kernels, key=lambda k: kernels[k]["avg_launch_us"])
     result = {
         "metric": METRIC,
         "value": round(mpix, 2),
@@ -311,11 +397,10 @@ def run_batch(args, preset, n_streams, ref_frames):
             "value": round(plain, 2), "unit": "Mpix/s",
             "what": "same 64 streams and frames through the one-stream drop-in, apply + synchronize per stream"},
         "batched_over_plain": None if plain is None else round(mpix / plain, 3),
-        "roofline": dict({"bound": "hbm", "kernel": "kb_match (speculative ordered sample match, all streams)",
-                          "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                          "frac": round(achieved / HBM_PEAK_GBS, 4)},
-                         **traffic_fields("subsense_batch64", bytes_launch),
-                         bytes_per_launch=int(bytes_launch), avg_launch_us=round(avg_match_s * 1e6, 2)),
+        "roofline": dict(kernels[dominant], **traffic_fields("subsense_batch64_" + dominant,
+                                                              kernels[dominant]["bytes_per_launch"])),
+        "kernels": {k: dict(v, **traffic_fields("subsense_batch64_" + k, v["bytes_per_launch"]))
+                    for k, v in kernels.items()},
         "frame_latency_ms": {"p50": round(float(np.percentile(step_ms, 50)), 4),
                              "p99": round(float(np.percentile(step_ms, 99)), 4),
                              "what": "GPU time of one batched step (every stream's frame), HIP events, %d steps"
@@ -326,10 +411,15 @@ def run_batch(args, preset, n_streams, ref_frames):
     }
     if args.serve_seconds > 0:
         result["serving"] = {}
-        for n in [int(x) for x in str(args.serve_streams).split(",") if x]:
-            log("serving: %d stream slots, %.0f s warm-up + %.0f s measured" % (n, args.serve_warmup,
-                                                                                args.serve_seconds))
-            result["serving"]["%d_streams" % n] = run_serve(args, n)
+        runs = [(int(x), 250.0, "device") for x in str(args.serve_streams).split(",") if x]
+        runs += [(int(x.split(":")[0]), float(x.split(":")[1]), "device") for x in str(args.serve_bound).split(",")
+                 if x]
+        runs += [(int(x), 250.0, "host") for x in str(args.serve_host).split(",") if x]
+        for n, gb, ingest in runs:
+            log("serving: %d stream slots, %.0f GB budget, %s frames, %.0f s warm-up + %.0f s measured" % (
+                n, gb, ingest, args.serve_warmup, args.serve_seconds))
+            key = "%d_streams" % n + ("" if gb == 250.0 else "_%dGB" % gb) + ("_host_ingest" if ingest == "host" else "")
+            result["serving"][key] = run_serve(args, n, gb, ingest)
         log("serving: %s" % json.dumps(result["serving"]))
     if not args.no_cpu:
         # streams 16g .. 16g+threads-1 of each resolution class g, one per thread, first frames
@@ -350,21 +440,32 @@ def run_batch(args, preset, n_streams, ref_frames):
 # serving: streams arrive and leave, frames arrive as Poisson processes (wall clock), the server
 # batches whatever is waiting
 # ------------------------------------------------------------------------------------------------
-def run_serve(args, n_slots, log_prefix="serve"):
+def run_serve(args, n_slots, budget_gb=250.0, ingest="device", log_prefix="serve"):
     """n_slots stream slots; resolutions from the SERVE generator preset ({320x240, 640x480, 1280x720,
     1920x1080} RGB with p = .3/.3/.2/.2); frames arrive at 30 fps (Poisson) per stream; lifetimes are
     exponential with a mean of 2000 frames and an ended stream is replaced by a new one; models are
-    admitted against a 250 GB budget (streams that do not fit wait). Latency = frame arrival ->
-    completion seen by the host. Frames come from a per-stream ring of pre-generated device frames."""
+    admitted against a device-memory budget (250 GB by default; streams that do not fit wait).
+    Latency = frame arrival -> completion seen by the host. Each stream plays its first `ring`
+    generated frames forwards and backwards (0, 1, .., ring-1, ring-2, .., 1, 0, 1, ..), so its
+    objects move continuously for the whole lifetime with no jump back. ingest = "device": the frames
+    are in HBM and handed over as device pointers; "host": they are in pinned host memory and go
+    through the server's ingest path (copied on its copy stream into the stream's device slots as
+    they arrive), so the latency includes the host-to-device copy."""
     import heapq
 
     import torch
     from litiv_amd import Server, synth
 
-    fps, mean_life, ring = 30.0, 2000.0, 16
+    from litiv_amd.bgs import PinnedBuffer
+
+    host = ingest == "host"
+    fps, mean_life = 30.0, 2000.0
+    ring = args.serve_host_ring if host else args.serve_ring
+    period = 2 * ring - 2
     rng = np.random.default_rng(0x5E12E)
-    srv = Server(hbm_budget=250 << 30, max_inflight=2, max_batch=n_slots, fix_px_per_block=8192,
-                 cont_px_per_block=2048, seq_limit=8, groups=2)
+    srv = Server(hbm_budget=int(budget_gb * (1 << 30)), max_inflight=2, max_batch=n_slots, fix_px_per_block=8192,
+                 cont_px_per_block=2048, seq_limit=8, groups=2, host_ring=4 if host else 0)
+    backlog = []    # host frames refused while their stream's ingest slots were busy: (arrival, sid, index, tag)
     dev = torch.device("cuda:0")
     incarnation = [0]
     slots = {}      # server stream id -> dict
@@ -384,10 +485,20 @@ def run_serve(args, n_slots, log_prefix="serve"):
         frames = torch.empty((ring, seq.frame_bytes), dtype=torch.uint8, device=dev)
         seq.frames_device(0, ring, frames.data_ptr())
         torch.cuda.synchronize()
-        sid = srv.open(frames.data_ptr(), st["w"], st["h"], 3, stream_id=st["k"])
+        if host:
+            pinned = PinnedBuffer(ring * seq.frame_bytes)
+            pinned.array[:] = frames.view(-1).cpu().numpy()
+            del frames
+            frames = pinned
+            sid = srv.open_host(pinned.array[:seq.frame_bytes].reshape(st["h"], st["w"], 3), stream_id=st["k"])
+        else:
+            sid = srv.open(frames.data_ptr(), st["w"], st["h"], 3, stream_id=st["k"])
         if sid is None:
+            if host:
+                frames.free()
             return None
-        st.update(sid=sid, frames=frames, masks=torch.empty((ring, st["w"] * st["h"]), dtype=torch.uint8, device=dev),
+        st.update(sid=sid, frames=frames, pend=[], ending=False,
+                  masks=torch.empty((ring, st["w"] * st["h"]), dtype=torch.uint8, device=dev),
                   t=0, done=0, closing=False, next=now + rng.exponential(1.0 / fps))
         slots[sid] = st
         return sid
@@ -421,18 +532,36 @@ def run_serve(args, n_slots, log_prefix="serve"):
             st = slots.get(sid)
             if st is None or st["closing"] or st.get("next") != ta:
                 continue
-            i = st["t"] % ring
+            i = st["t"] % period
+            i = i if i < ring else period - i
             tag_seq += 1
-            srv.submit(sid, st["frames"][i].data_ptr(), st["masks"][i].data_ptr(), 0.0, tag=tag_seq)
             arrival[tag_seq] = ta
             npx[tag_seq] = st["w"] * st["h"]
+            if host:
+                st["pend"].append((i, tag_seq))
+            else:
+                srv.submit(sid, st["frames"][i].data_ptr(), st["masks"][i].data_ptr(), 0.0, tag=tag_seq)
             st["t"] += 1
             if st["t"] >= st["life"]:
-                st["closing"] = True
-                srv.close(sid)
+                st["ending"] = True
             else:
                 st["next"] = ta + rng.exponential(1.0 / fps)
                 heapq.heappush(heap, (st["next"], sid))
+        if host:
+            # host frames go to the ingest path in arrival order; a stream whose device slots are all
+            # busy keeps the rest waiting (their latency clock runs)
+            for sid, st in slots.items():
+                pend = st["pend"]
+                while pend:
+                    i, tag = pend[0]
+                    fb = st["seq"].frame_bytes
+                    if not srv.submit_host(sid, st["frames"].ptr + i * fb, st["masks"][i].data_ptr(), 0.0, tag=tag):
+                        break
+                    pend.pop(0)
+        for sid, st in slots.items():
+            if st.get("ending") and not st["closing"] and not st["pend"]:
+                st["closing"] = True
+                srv.close(sid)
         srv.pump()
         done = srv.poll()
         if len(done):
@@ -471,7 +600,10 @@ def run_serve(args, n_slots, log_prefix="serve"):
     lat_ms = np.array(lat) * 1e3
     offered = sum(st["w"] * st["h"] for st in slots.values()) * fps / 1e6
     return {
-        "streams": n_slots, "open_at_end": sst["open_streams"], "waiting_at_end": len(waiting),
+        "streams": n_slots, "budget_gb": budget_gb, "frames": "%d-frame ping-pong per stream" % ring,
+        "ingest": "pinned host memory through the server's copy stream (latency includes the copy)" if host
+                  else "device frames",
+        "open_at_end": sst["open_streams"], "waiting_at_end": len(waiting),
         "replaced": replaced, "window_s": round(window, 1),
         "latency_ms": {"p50": round(float(np.percentile(lat_ms, 50)), 3) if len(lat_ms) else None,
                        "p99": round(float(np.percentile(lat_ms, 99)), 3) if len(lat_ms) else None,
@@ -548,6 +680,15 @@ def run_plain(args, name, algo, preset, ref_frames):
         iters.append(st["fixpoint_iters"])
         work += np.array(st["fixpoint_work"], dtype=np.float64)
         max_iter = max(max_iter, st["max_fixpoint_iter"])
+    upd_ms, upd_wr = [], 0
+    bgs.set_timing("update")
+    warm()
+    for i in range(n_meas):
+        t = args.warmup + i
+        bgs.apply_device(base + t * fb, masks[t % 8].data_ptr(), lr)
+        ms, _ = bgs.take_timing()
+        upd_ms.append(ms)
+        upd_wr += bgs.stats()["writes"]
     bgs.set_timing("step")
     warm()
     step_ms = []
@@ -558,9 +699,15 @@ def run_plain(args, name, algo, preset, ref_frames):
         step_ms.append(ms)
     bgs.set_timing("")
     n_roi = st["n_roi"]
-    avg_match_s = float(np.mean(match_ms)) / 1e3
-    bytes_match = match_bytes(C, n_roi, ex_spec / n_meas)
-    achieved = bytes_match / avg_match_s / 1e9
+    ub = (update_bytes if algo == "subsense" else update_bytes_lobster)(C, n_roi, upd_wr / n_meas)
+    kernels = {
+        "match": dict(kernel_figures("k_match (phase A of the ordered sample match)", float(np.mean(match_ms)) / 1e3,
+                                     match_bytes(C, n_roi, ex_spec / n_meas)),
+                      examined_per_px=round(ex_spec / n_meas / n_roi, 3)),
+        "update": kernel_figures("k_update (model update and diffusion%s)" % (
+            ", feedback maps fused" if algo == "subsense" else ""), float(np.mean(upd_ms)) / 1e3, ub),
+    }
+    dominant = max(kernels, key=lambda k: kernels[k]["avg_launch_us"])
     sb = step_bytes(C, n_roi, ex_fin / n_meas, wr / n_meas)
     result = {
         "metric": METRIC,
@@ -577,12 +724,9 @@ def run_plain(args, name, algo, preset, ref_frames):
         "data": "synthetic (deterministic generator, frames resident in HBM)",
         "config": {"workload": name, "width": W, "height": H, "channels": C, "frames": total,
                    "algo": algo, "learning_rate": lr, "streams": 1, "parallelism": "single stream, one GPU"},
-        "roofline": dict({"bound": "hbm", "kernel": "k_match (speculative ordered sample match)",
-                          "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                          "frac": round(achieved / HBM_PEAK_GBS, 4)},
-                         **traffic_fields(name, bytes_match),
-                         bytes_per_launch=int(bytes_match), avg_launch_us=round(avg_match_s * 1e6, 2),
-                         examined_per_px=round(ex_spec / n_meas / n_roi, 3)),
+        "roofline": dict(kernels[dominant], **traffic_fields(name + "_" + dominant,
+                                                              kernels[dominant]["bytes_per_launch"])),
+        "kernels": {k: dict(v, **traffic_fields(name + "_" + k, v["bytes_per_launch"])) for k, v in kernels.items()},
         "frame_latency_ms": {"p50": round(float(np.percentile(step_ms, 50)), 4),
                              "p99": round(float(np.percentile(step_ms, 99)), 4),
                              "what": "GPU time of one frame step (HIP events), %d frames" % n_meas},
@@ -610,11 +754,21 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="batched workload: skip the serial plain comparison")
     ap.add_argument("--no-tune", action="store_true", help="batched workload: default launch options")
+    ap.add_argument("--tune-steps", type=int, default=144,
+                    help="batched workload: steps of the tuning pass run before the timed region")
+    ap.add_argument("--tune-period", type=int, default=8, help="tuning pass: steps between trial rounds")
     ap.add_argument("--calib-copy-mb", type=int, default=0, help="profiling runs: one device copy of this size first")
     ap.add_argument("--serve-seconds", type=float, default=20.0,
                     help="batched workload: measured seconds of the serving run (0 = skip)")
     ap.add_argument("--serve-warmup", type=float, default=5.0)
     ap.add_argument("--serve-streams", default="256,200", help="comma-separated stream-slot counts, one run each")
+    ap.add_argument("--serve-bound", default="256:100",
+                    help="comma-separated slots:GB serving runs with a smaller model budget (admission binds)")
+    ap.add_argument("--serve-ring", type=int, default=32, help="generated frames per served stream (ping-pong)")
+    ap.add_argument("--serve-host", default="256",
+                    help="comma-separated stream-slot counts of serving runs fed from pinned host memory")
+    ap.add_argument("--serve-host-ring", type=int, default=8,
+                    help="host-fed serving: pinned frames per stream (ping-pong)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; multi-GPU runs (--gpus %d) are not part of it. Streams "

@@ -26,6 +26,7 @@ extern "C" {
 #define LVBGS_ERR_ALLOC (-4)    /* device memory allocation failed */
 #define LVBGS_ERR_DEVICE (-5)   /* device-side consistency check failed */
 #define LVBGS_ERR_BUDGET (-6)   /* a server's device-memory budget cannot hold the model */
+#define LVBGS_ERR_BUSY (-7)     /* a served stream's host-ingest slots are all busy: poll, then retry */
 
 enum { LVBGS_SUBSENSE = 1, LVBGS_LOBSTER = 2 };
 
@@ -221,6 +222,8 @@ typedef struct lvbgs_server_options {
     int max_inflight;             /* steps queued on the device at once (1..16) */
     int max_batch;                /* streams per step at most */
     lvbgs_batch_options launch;   /* launch options of each step (use_graph is ignored) */
+    int host_ring;                /* 0..64 device frame slots per stream for lvbgs_server_submit_host, held in
+                                     the stream's arena and counted in the budget (0: device frames only) */
 } lvbgs_server_options;
 typedef struct lvbgs_done {
     uint64_t tag;                 /* the submit call's tag */
@@ -250,6 +253,25 @@ int lvbgs_server_open(lvbgs_server* s, const lvbgs_params* params, const void* d
  * On a stream the server closed after a device error it acknowledges the failure (LVBGS_OK). */
 int lvbgs_server_close(lvbgs_server* s, int stream);
 int lvbgs_server_submit(lvbgs_server* s, int stream, const void* d_img, void* d_mask, double lr, uint64_t tag);
+/* Ingest from host memory (no counterpart in the reference, whose datasets module prefetches frames
+ * on a host thread, modules/datasets/include/litiv/datasets/utils.hpp:456 DataPrecacher): the frame
+ * (row stride in bytes, 0 = packed) is copied at once into the stream's next free device slot on the
+ * server's copy stream, so the copy overlaps the steps already on the device; the step that takes the
+ * frame waits for its copy on the device. Needs host_ring > 0. host_img must stay valid and unchanged
+ * until the copy has run (at the latest, until the frame completes); pinned memory (lvbgs_host_alloc)
+ * makes the copy asynchronous. LVBGS_ERR_BUSY when all the stream's slots hold frames not completed. */
+int lvbgs_server_submit_host(lvbgs_server* s, int stream, const void* host_img, size_t stride, void* d_mask,
+                             double lr, uint64_t tag);
+/* lvbgs_server_open from a frame in host memory (row stride in bytes, 0 = packed) */
+int lvbgs_server_open_host(lvbgs_server* s, const lvbgs_params* params, const void* img0, size_t stride, int width,
+                           int height, int channels, int* stream_out);
+/* device bytes a SuBSENSE stream of this geometry holds in this server (what admission counts against
+ * the budget: lvbgs_model_bytes plus its host-ingest slots); 0 for invalid arguments */
+size_t lvbgs_server_stream_bytes(const lvbgs_server* s, const lvbgs_params* params, int width, int height,
+                                 int channels);
+/* pinned (page-locked) host memory for frames handed to lvbgs_server_submit_host; NULL on failure */
+void* lvbgs_host_alloc(size_t bytes);
+void lvbgs_host_free(void* p);
 /* non-blocking: collects finished steps, launches steps while the device queue has room;
  * returns the number of steps launched or < 0 */
 int lvbgs_server_pump(lvbgs_server* s);
@@ -283,6 +305,26 @@ typedef struct lvbgs_binclassif {
  * buffer per call (about 8 bytes per 1024 pixels). */
 int lvbgs_binclassif_accumulate(const void* d_classif, const void* d_gt, const void* d_roi, int width, int height,
                                 size_t stride, lvbgs_binclassif* counts, void* stream);
+
+/* ---- binary matrix archives and checkpoint / resume ----
+ * lv::write / lv::read with lv::MatArchive_BINARY (modules/utils/src/opencv.cpp:514-531, :608-625):
+ * int32 OpenCV type (depth | (channels - 1) << 3), uint64 element size, uint64 element count,
+ * int32 dimension count, int32 sizes, then the dense elements. Files written here are read by the
+ * reference's lv::read and the other way round. */
+int lvbgs_archive_write(const char* path, int cv_type, int dims, const int32_t* sizes, const void* data);
+/* Reads the header (type, dims, up to max_dims sizes) and, with data != NULL, the elements (at most
+ * cap bytes). Returns the element bytes, or LVBGS_ERR_ARG for a missing / truncated / inconsistent
+ * file or elements larger than cap. */
+long long lvbgs_archive_read(const char* path, int* cv_type, int* dims, int32_t* sizes, int max_dims, void* data,
+                             size_t cap);
+/* Writes the whole model of an initialized context into the existing directory dir, one archive per
+ * state buffer (the getters' layouts) plus meta.bin and lut.bin. The reference keeps its model only
+ * in memory (BackgroundSubtractorSuBSENSE.hpp:70-128); a checkpoint lets a stream resume after a
+ * restart. */
+int lvbgs_checkpoint_save(lvbgs_ctx* ctx, const char* dir);
+/* Restores a checkpoint into a context created with the same algorithm, parameters, rng_seed and
+ * stream_id (else LVBGS_ERR_ARG): the context then continues the saved sequence bit for bit. */
+int lvbgs_checkpoint_load(lvbgs_ctx* ctx, const char* dir);
 
 #ifdef __cplusplus
 }

@@ -17,6 +17,8 @@ import os
 
 import numpy as np
 
+from . import _native
+
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "liblitiv_bgs.so")
 
@@ -64,7 +66,7 @@ class BatchOptions(C.Structure):
 
 class ServerOptions(C.Structure):
     _fields_ = [("hbm_budget", C.c_size_t), ("max_inflight", C.c_int), ("max_batch", C.c_int),
-                ("launch", BatchOptions)]
+                ("launch", BatchOptions), ("host_ring", C.c_int)]
 
 
 class BinClassifCounts(C.Structure):
@@ -80,6 +82,7 @@ class ServerStats(C.Structure):
 
 DONE_DTYPE = np.dtype([("tag", np.uint64), ("stream", np.int32), ("status", np.int32)])
 ERR_BUDGET = -6
+ERR_BUSY = -7
 
 _lib = None
 
@@ -95,7 +98,7 @@ def lib():
         if not os.path.exists(_LIB_PATH):
             raise RuntimeError("liblitiv_bgs.so is not built (run __graft_entry__.build()); "
                                "litiv_amd has no CPU fallback")
-        L = C.CDLL(_LIB_PATH)
+        L = _native.load(_LIB_PATH)
         vp, sz, i = C.c_void_p, C.c_size_t, C.c_int
         L.lvbgs_default_params.argtypes = [i, C.POINTER(Params)]
         L.lvbgs_create.argtypes = [i, C.POINTER(Params), C.POINTER(vp)]
@@ -141,6 +144,13 @@ def lib():
         L.lvbgs_server_open.argtypes = [vp, C.POINTER(Params), vp, i, i, i, C.POINTER(i)]
         L.lvbgs_server_close.argtypes = [vp, i]
         L.lvbgs_server_submit.argtypes = [vp, i, vp, vp, C.c_double, C.c_uint64]
+        L.lvbgs_server_submit_host.argtypes = [vp, i, vp, sz, vp, C.c_double, C.c_uint64]
+        L.lvbgs_server_open_host.argtypes = [vp, C.POINTER(Params), vp, sz, i, i, i, C.POINTER(i)]
+        L.lvbgs_server_stream_bytes.argtypes = [vp, C.POINTER(Params), i, i, i]
+        L.lvbgs_server_stream_bytes.restype = sz
+        L.lvbgs_host_alloc.argtypes = [sz]
+        L.lvbgs_host_alloc.restype = vp
+        L.lvbgs_host_free.argtypes = [vp]
         L.lvbgs_server_pump.argtypes = [vp]
         L.lvbgs_server_poll.argtypes = [vp, vp, i]
         L.lvbgs_server_drain.argtypes = [vp]
@@ -149,6 +159,12 @@ def lib():
         L.lvbgs_server_context.restype = vp
         L.lvbgs_server_debug_fixpoint_cap.argtypes = [vp, i]
         L.lvbgs_binclassif_accumulate.argtypes = [vp, vp, vp, i, i, C.c_size_t, C.POINTER(BinClassifCounts), vp]
+        L.lvbgs_archive_write.argtypes = [C.c_char_p, i, i, C.POINTER(C.c_int32), vp]
+        L.lvbgs_archive_read.argtypes = [C.c_char_p, C.POINTER(i), C.POINTER(i), C.POINTER(C.c_int32), i, vp,
+                                         C.c_size_t]
+        L.lvbgs_archive_read.restype = C.c_longlong
+        L.lvbgs_checkpoint_save.argtypes = [vp, C.c_char_p]
+        L.lvbgs_checkpoint_load.argtypes = [vp, C.c_char_p]
         L.lvbgs_batch_create.argtypes = [C.POINTER(vp), i, C.POINTER(vp)]
         L.lvbgs_batch_destroy.argtypes = [vp]
         L.lvbgs_batch_last_error.argtypes = [vp]
@@ -188,6 +204,44 @@ def lbsp_compute_abs(d_img, width, height, channels, threshold, d_out, stream=No
                                       C.c_void_p(stream or 0))
     if rc != OK:
         raise LvbgsError(rc, "lvbgs_lbsp_compute_abs failed")
+
+
+# OpenCV depth codes of the archive format and the numpy types they hold
+_CV_DEPTHS = {0: np.uint8, 1: np.int8, 2: np.uint16, 3: np.int16, 4: np.int32, 5: np.float32, 6: np.float64}
+
+
+def write_archive(path, arr, channels=1):
+    """lv::write(path, mat, lv::MatArchive_BINARY): arr's trailing axis holds the channels when
+    channels > 1 (an H x W x 3 array with channels=3 is a 2-D CV_8UC3 matrix)."""
+    arr = np.ascontiguousarray(arr)
+    depth = {np.dtype(v): k for k, v in _CV_DEPTHS.items()}.get(arr.dtype)
+    if depth is None:
+        raise TypeError("unsupported element type %s" % arr.dtype)
+    shape = arr.shape[:-1] if channels > 1 else arr.shape
+    if channels > 1 and arr.shape[-1] != channels:
+        raise ValueError("trailing axis must hold the channels")
+    sizes = (C.c_int32 * max(1, len(shape)))(*shape)
+    rc = lib().lvbgs_archive_write(os.fsencode(path), depth + ((channels - 1) << 3), len(shape), sizes,
+                                   arr.ctypes.data)
+    if rc:
+        raise LvbgsError(rc, "cannot write archive %s" % path)
+
+
+def read_archive(path):
+    """lv::read(path, lv::MatArchive_BINARY): the matrix as a numpy array (channels on a trailing
+    axis when there are more than one) and its OpenCV type code."""
+    t, nd, sizes = C.c_int(), C.c_int(), (C.c_int32 * 8)()
+    n = lib().lvbgs_archive_read(os.fsencode(path), C.byref(t), C.byref(nd), sizes, 8, None, 0)
+    if n < 0:
+        raise LvbgsError(int(n), "cannot read archive %s" % path)
+    depth, ch = t.value & 7, (t.value >> 3) + 1
+    if depth not in _CV_DEPTHS or nd.value > 8:
+        raise LvbgsError(-1, "unsupported archive type %d in %s" % (t.value, path))
+    shape = tuple(sizes[:nd.value]) + ((ch,) if ch > 1 else ())
+    out = np.empty(shape, _CV_DEPTHS[depth])
+    if lib().lvbgs_archive_read(os.fsencode(path), None, None, None, 0, out.ctypes.data, out.nbytes) != n:
+        raise LvbgsError(-1, "cannot read archive %s" % path)
+    return out, t.value
 
 
 class BackgroundSubtractor:
@@ -349,6 +403,19 @@ class BackgroundSubtractor:
             raise KeyError(name)
         return out
 
+    # ---- checkpoint / resume ----
+    def save_checkpoint(self, directory):
+        """Writes the whole model into directory (created if missing), one binary archive per state
+        buffer plus meta.bin; load_checkpoint on a subtractor created with the same parameters
+        continues the sequence exactly."""
+        os.makedirs(directory, exist_ok=True)
+        self._check(lib().lvbgs_checkpoint_save(self._h, os.fsencode(directory)))
+
+    def load_checkpoint(self, directory):
+        self._check(lib().lvbgs_checkpoint_load(self._h, os.fsencode(directory)))
+        meta, _ = read_archive(os.path.join(directory, "meta.bin"))
+        self.shape = (int(meta[3]), int(meta[2]), int(meta[4]))
+
     def set_timing(self, stage):
         self._check(lib().lvbgs_set_timing(self._h, (stage or "").encode()))
 
@@ -498,12 +565,14 @@ class Server:
     """Serving loop over resident SuBSENSE models (continuous batching, device-memory admission).
     Frames and masks are device pointers; completions come back through poll()."""
 
-    def __init__(self, device=0, hbm_budget=250 << 30, max_inflight=2, max_batch=256, **launch):
+    def __init__(self, device=0, hbm_budget=250 << 30, max_inflight=2, max_batch=256, host_ring=0, **launch):
         o = ServerOptions()
         lib().lvbgs_server_default_options(C.byref(o))
         o.hbm_budget = int(hbm_budget)
         o.max_inflight = int(max_inflight)
         o.max_batch = int(max_batch)
+        o.host_ring = int(host_ring)
+        self.host_ring = int(host_ring)
         for k, v in launch.items():
             if not hasattr(o.launch, k):
                 raise TypeError("unknown launch option %r" % k)
@@ -537,6 +606,14 @@ class Server:
             setattr(p, k, v)
         return lib().lvbgs_model_bytes(SUBSENSE, C.byref(p), width, height, channels)
 
+    def stream_bytes(self, width, height, channels, **params):
+        """Device bytes a stream of this geometry holds in this server (model, per-step buffers and its
+        host-ingest slots): what admission counts against the budget."""
+        p = default_params(SUBSENSE)
+        for k, v in params.items():
+            setattr(p, k, v)
+        return lib().lvbgs_server_stream_bytes(self._h, C.byref(p), width, height, channels)
+
     def open(self, d_img0, width, height, channels, **params):
         """Admits a stream; returns its id, or None when its model does not fit the budget."""
         p = default_params(SUBSENSE)
@@ -552,6 +629,36 @@ class Server:
         self._params[sid.value] = p
         self._shape[sid.value] = (height, width, channels)
         return sid.value
+
+    def open_host(self, img0, **params):
+        """Admits a stream initialized from a host frame (numpy HxWxC / HxW uint8); id or None."""
+        img0 = _u8(img0)
+        h, w = img0.shape[:2]
+        c = 1 if img0.ndim == 2 else img0.shape[2]
+        p = default_params(SUBSENSE)
+        for k, v in params.items():
+            if not hasattr(p, k):
+                raise TypeError("unknown parameter %r" % k)
+            setattr(p, k, v)
+        sid = C.c_int()
+        rc = lib().lvbgs_server_open_host(self._h, C.byref(p), img0.ctypes.data, img0.strides[0], w, h, c,
+                                          C.byref(sid))
+        if rc == ERR_BUDGET:
+            return None
+        self._check(rc)
+        self._params[sid.value] = p
+        self._shape[sid.value] = (h, w, c)
+        return sid.value
+
+    def submit_host(self, sid, host_ptr, d_mask=None, lr=0.0, tag=0, stride=0):
+        """Submits a frame in host memory (address host_ptr, kept unchanged until it completes);
+        returns False when the stream's ingest slots are all busy (poll, then retry)."""
+        rc = lib().lvbgs_server_submit_host(self._h, int(sid), C.c_void_p(host_ptr), int(stride),
+                                            C.c_void_p(d_mask or 0), float(lr), int(tag))
+        if rc == ERR_BUSY:
+            return False
+        self._check(rc)
+        return True
 
     def close(self, sid):
         self._check(lib().lvbgs_server_close(self._h, int(sid)))
@@ -649,3 +756,28 @@ class BinClassif:
             "fmeasure": 2.0 * (recall * precision) / (recall + precision) if recall + precision > 0 else 0.0,
             "mcc": mcc,
         }
+
+
+class PinnedBuffer:
+    """Page-locked host memory (lvbgs_host_alloc) viewed as a numpy uint8 array, for frames submitted
+    with Server.submit_host (asynchronous copies)."""
+
+    def __init__(self, nbytes):
+        self._p = lib().lvbgs_host_alloc(int(nbytes))
+        if not self._p:
+            raise LvbgsError(ERR_ALLOC, "lvbgs_host_alloc failed")
+        self.nbytes = int(nbytes)
+        self.array = np.ctypeslib.as_array((C.c_uint8 * self.nbytes).from_address(self._p))
+
+    @property
+    def ptr(self):
+        return self._p
+
+    def free(self):
+        if getattr(self, "_p", None):
+            self.array = None
+            lib().lvbgs_host_free(self._p)
+            self._p = None
+
+    def __del__(self):
+        self.free()

@@ -18,7 +18,7 @@ HIP_FLAGS = [
 ]
 
 TARGETS = {
-    "liblitiv_bgs.so": ["bgs_kernels.hip", "bgs_api.hip", "bgs_serve.hip", "bgs_eval.hip"],
+    "liblitiv_bgs.so": ["bgs_kernels.hip", "bgs_api.hip", "bgs_serve.hip", "bgs_eval.hip", "bgs_archive.hip"],
     "liblitiv_synth.so": ["synth.hip"],
 }
 

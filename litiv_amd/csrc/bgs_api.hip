@@ -178,12 +178,14 @@ static void carve(lvbgs_ctx* c, Arena& ar) {
     v.sc = ar.take<Scalars>(1);
     c->d_frame = ar.take<uint8_t>(npx * C);
     c->d_mask = ar.take<uint8_t>(npx);
+    c->d_ring = c->ring_frames > 0 ? ar.take<uint8_t>((size_t)c->ring_frames * npx * C) : nullptr;
 }
 
-size_t lvbgs_arena_bytes(int algo, const lvbgs_params& p, int width, int height, int channels) {
+size_t lvbgs_arena_bytes(int algo, const lvbgs_params& p, int width, int height, int channels, int ring_frames) {
     lvbgs_ctx c;
     c.algo = algo;
     c.p = p;
+    c.ring_frames = ring_frames;
     c.W = width;
     c.H = height;
     c.C = channels;
@@ -230,6 +232,75 @@ static int download_scalars(lvbgs_ctx* c, Scalars& s) {
     return hip_check(c, hipStreamSynchronize(c->stream), "sync");
 }
 
+// Geometry, arena and View of a context initialized on a w x h x ch frame whose ROI (already in
+// c->roi) counts orig pixels before and fin after border trimming; fills s with the initial
+// frame-level scalars (LUT, caps, automatic reset) of BackgroundSubtractorLBSP.cpp:29-43 and
+// BackgroundSubtractorSuBSENSE.cpp:112-128.
+static int setup_geometry(lvbgs_ctx* c, int w, int h, int ch, size_t orig, size_t fin, Scalars& s) {
+    const size_t npx = (size_t)w * h;
+    c->W = w;
+    c->H = h;
+    c->C = ch;
+    c->npx = npx;
+    c->orig_roi = orig;
+    c->n_roi = fin;
+    c->apply_count = 0;
+    int rc = allocate(c);
+    if (rc) return rc;
+    View& v = c->v;
+    v.W = w;
+    v.H = h;
+    v.C = ch;
+    v.N = (int)c->p.bg_samples;
+    v.REQ = (int)c->p.required_bg_samples;
+    v.algo = c->algo;
+    v.npx = (uint32_t)npx;
+    v.n_roi = (uint32_t)fin;
+    v.color_thr = (uint32_t)c->p.color_dist_threshold;
+    v.desc_thr = (uint32_t)c->p.desc_dist_threshold;
+    v.mv_avgs = (uint32_t)c->p.samples_for_mv_avgs;
+    v.lbsp_offset = (uint32_t)c->p.lbsp_threshold_offset;
+    v.rel_lbsp = c->p.rel_lbsp_threshold;
+    v.seed_lo = (uint32_t)c->p.rng_seed;
+    v.seed_hi = (uint32_t)(c->p.rng_seed >> 32);
+    v.stream_id = c->p.stream_id;
+    v.dsW = w / 8;
+    v.dsH = h / 8;
+
+    std::memset(&s, 0, sizeof(s));
+    // LBSP LUT (BackgroundSubtractorLBSP.cpp:29-30, :42-43)
+    for (int t = 0; t <= 255; ++t) {
+        const float base = (float)t * c->p.rel_lbsp_threshold + (float)c->p.lbsp_threshold_offset;
+        s.lut[t] = sat_u8_host(ch == 1 ? base / 3.0f : base);
+    }
+    float lower = 2.0f, upper = 256.0f;
+    if (c->algo == LVBGS_SUBSENSE) {
+        // SuBSENSE .cpp:112-128
+        if (orig >= npx / 2 && (int)npx >= 320 * 240) {
+            v.lr_scaling = 1;
+            s.auto_reset = 1;
+            v.use3x3 = !((int)npx > 320 * 240 * 2);
+            const int raw = (int)std::floor((float)npx / (float)(320 * 240) + 0.5f) + c->p.median_blur_kernel;
+            const int rk = raw < 14 ? raw : 14;
+            v.median_k = (rk % 2) ? rk : rk - 1;
+        } else {
+            v.lr_scaling = 0;
+            s.auto_reset = 0;
+            v.use3x3 = 1;
+            v.median_k = c->p.median_blur_kernel;
+            lower = 4.0f;
+            upper = 512.0f;
+        }
+    } else {
+        v.lr_scaling = 0;
+        v.use3x3 = 1;
+        v.median_k = c->p.median_blur_kernel;
+    }
+    s.lower_cap = lower;
+    s.upper_cap = upper;
+    return LVBGS_OK;
+}
+
 // done == nullptr: synchronous (the reference's contract); otherwise the work stays queued on the
 // context's stream and `done` is recorded after it
 static int initialize(lvbgs_ctx* c, const uint8_t* img, bool device_img, int w, int h, int ch, size_t stride,
@@ -249,6 +320,10 @@ extern "C" int lvbgs_initialize_device(lvbgs_ctx* c, const void* d_img, int w, i
 int lvbgs_initialize_async(lvbgs_ctx* c, const void* d_img, int w, int h, int ch, hipEvent_t done) {
     return initialize(c, (const uint8_t*)d_img, true, w, h, ch, (size_t)w * (size_t)(ch > 0 ? ch : 1), nullptr, 0,
                       done);
+}
+
+int lvbgs_initialize_async_host(lvbgs_ctx* c, const void* img, int w, int h, int ch, size_t stride, hipEvent_t done) {
+    return initialize(c, (const uint8_t*)img, false, w, h, ch, stride, nullptr, 0, done);
 }
 
 static int initialize(lvbgs_ctx* c, const uint8_t* img, bool device_img, int w, int h, int ch, size_t stride,
@@ -312,70 +387,13 @@ static int initialize(lvbgs_ctx* c, const uint8_t* img, bool device_img, int w, 
     }
 
     c->initialized = false;
-    c->W = w;
-    c->H = h;
-    c->C = ch;
-    c->npx = npx;
-    c->orig_roi = orig;
-    c->n_roi = fin;
-    if (!reuse_roi) c->roi = std::move(nr);
     c->roi_w = w;
     c->roi_h = h;
-    c->apply_count = 0;
-    int rc = allocate(c);
+    if (!reuse_roi) c->roi = std::move(nr);
+    Scalars s;
+    int rc = setup_geometry(c, w, h, ch, orig, fin, s);
     if (rc) return rc;
     View& v = c->v;
-    v.W = w;
-    v.H = h;
-    v.C = ch;
-    v.N = (int)c->p.bg_samples;
-    v.REQ = (int)c->p.required_bg_samples;
-    v.algo = c->algo;
-    v.npx = (uint32_t)npx;
-    v.n_roi = (uint32_t)fin;
-    v.color_thr = (uint32_t)c->p.color_dist_threshold;
-    v.desc_thr = (uint32_t)c->p.desc_dist_threshold;
-    v.mv_avgs = (uint32_t)c->p.samples_for_mv_avgs;
-    v.lbsp_offset = (uint32_t)c->p.lbsp_threshold_offset;
-    v.rel_lbsp = c->p.rel_lbsp_threshold;
-    v.seed_lo = (uint32_t)c->p.rng_seed;
-    v.seed_hi = (uint32_t)(c->p.rng_seed >> 32);
-    v.stream_id = c->p.stream_id;
-    v.dsW = w / 8;
-    v.dsH = h / 8;
-
-    Scalars s;
-    std::memset(&s, 0, sizeof(s));
-    // LBSP LUT (BackgroundSubtractorLBSP.cpp:29-30, :42-43)
-    for (int t = 0; t <= 255; ++t) {
-        const float base = (float)t * c->p.rel_lbsp_threshold + (float)c->p.lbsp_threshold_offset;
-        s.lut[t] = sat_u8_host(ch == 1 ? base / 3.0f : base);
-    }
-    float lower = 2.0f, upper = 256.0f;
-    if (c->algo == LVBGS_SUBSENSE) {
-        // SuBSENSE .cpp:112-128
-        if (orig >= npx / 2 && (int)npx >= 320 * 240) {
-            v.lr_scaling = 1;
-            s.auto_reset = 1;
-            v.use3x3 = !((int)npx > 320 * 240 * 2);
-            const int raw = (int)std::floor((float)npx / (float)(320 * 240) + 0.5f) + c->p.median_blur_kernel;
-            const int rk = raw < 14 ? raw : 14;
-            v.median_k = (rk % 2) ? rk : rk - 1;
-        } else {
-            v.lr_scaling = 0;
-            s.auto_reset = 0;
-            v.use3x3 = 1;
-            v.median_k = c->p.median_blur_kernel;
-            lower = 4.0f;
-            upper = 512.0f;
-        }
-    } else {
-        v.lr_scaling = 0;
-        v.use3x3 = 1;
-        v.median_k = c->p.median_blur_kernel;
-    }
-    s.lower_cap = lower;
-    s.upper_cap = upper;
     if ((rc = upload_scalars(c, s))) return rc;
     if ((rc = hip_check(c, hipMemcpyAsync((void*)v.roi, c->roi.data(), npx, hipMemcpyHostToDevice, c->stream), "roi")))
         return rc;
@@ -388,7 +406,7 @@ static int initialize(lvbgs_ctx* c, const uint8_t* img, bool device_img, int w, 
     }
     if (c->algo == LVBGS_SUBSENSE) {
         const unsigned b = (unsigned)((npx + 255) / 256);
-        hipLaunchKernelGGL(k_fill_f32, dim3(b), dim3(256), 0, c->stream, v.T, npx, lower);
+        hipLaunchKernelGGL(k_fill_f32, dim3(b), dim3(256), 0, c->stream, v.T, npx, s.lower_cap);
         hipLaunchKernelGGL(k_fill_f32, dim3(b), dim3(256), 0, c->stream, v.R, npx, 1.0f);
         hipLaunchKernelGGL(k_fill_f32, dim3(b), dim3(256), 0, c->stream, v.V, npx, 10.0f);
     }
@@ -708,6 +726,110 @@ extern "C" long long lvbgs_get_state(lvbgs_ctx* c, const char* name, void* out, 
     if (hipStreamSynchronize(c->stream) != hipSuccess) return LVBGS_ERR_HIP;
     if (hipMemcpy(out, src, bytes, hipMemcpyDeviceToHost) != hipSuccess) return LVBGS_ERR_HIP;
     return (long long)bytes;
+}
+
+// ---- checkpoint restore (bgs_archive.hip): geometry first, then the saved buffers, then the scalars ----
+// samples are staged on the host in the device record layout until the scalars arrive
+static std::vector<uint32_t>& restore_records(lvbgs_ctx* c) {
+    static thread_local std::vector<std::pair<lvbgs_ctx*, std::vector<uint32_t>>> staged;
+    for (auto& e : staged)
+        if (e.first == c) return e.second;
+    staged.emplace_back(c, std::vector<uint32_t>());
+    return staged.back().second;
+}
+
+static void drop_restore_records(lvbgs_ctx* c) {
+    std::vector<uint32_t>& r = restore_records(c);
+    std::vector<uint32_t>().swap(r);
+}
+
+int lvbgs_restore_begin(lvbgs_ctx* c, int w, int h, int ch, const uint8_t* roi, size_t n_roi, size_t orig_roi) {
+    (void)hipSetDevice(c->device);
+    const size_t npx = (size_t)w * (size_t)h;
+    size_t fin = 0;
+    for (size_t i = 0; i < npx; ++i) fin += roi[i] ? 1 : 0;
+    if (fin != n_roi || !fin) return set_err(c, LVBGS_ERR_ARG, "checkpoint ROI does not match its pixel count");
+    c->initialized = false;
+    c->roi.assign(roi, roi + npx);
+    c->roi_w = w;
+    c->roi_h = h;
+    Scalars s;
+    int rc = setup_geometry(c, w, h, ch, orig_roi, fin, s);
+    if (rc) return rc;
+    if ((rc = hip_check(c, hipMemcpyAsync((void*)c->v.roi, c->roi.data(), npx, hipMemcpyHostToDevice, c->stream),
+                        "roi")))
+        return rc;
+    const size_t rec = c->C == 3 ? 3 : 1, N = c->p.bg_samples;
+    std::vector<uint32_t>& r = restore_records(c);
+    r.assign((N * ((npx + 63) / 64) + kMaxFetch) * 64 * rec, 0u);
+    return hip_check(c, hipStreamSynchronize(c->stream), "restore");
+}
+
+int lvbgs_restore_buffer(lvbgs_ctx* c, const char* name, const void* data, size_t bytes) {
+    const View& v = c->v;
+    const size_t npx = c->npx, C = (size_t)c->C, N = c->p.bg_samples;
+    const std::string n(name);
+    if (n == "samples_color" || n == "samples_desc") {
+        // the oracle layout (N x H x W x C) into the tile-major records (sample s of 64-pixel group g
+        // at record (g*N + s)*64 + lane); RGB {d0 | d1<<16, d2 | c0<<16 | c1<<24, c2}, grey {d0 | c0<<16}
+        std::vector<uint32_t>& r = restore_records(c);
+        const bool col = n == "samples_color";
+        if (bytes != N * npx * C * (col ? 1 : 2) || r.empty()) return set_err(c, LVBGS_ERR_ARG, "sample planes");
+        const size_t rw = C == 3 ? 3 : 1;
+        for (size_t i = 0; i < N * npx; ++i) {
+            const size_t s = i / npx, p = i % npx;
+            uint32_t* w = &r[(((p >> 6) * N + s) * 64 + (p & 63)) * rw];
+            if (C == 3) {
+                if (col) {
+                    const uint8_t* o = (const uint8_t*)data + i * 3;
+                    w[1] = (w[1] & 0xFFFFu) | ((uint32_t)o[0] << 16) | ((uint32_t)o[1] << 24);
+                    w[2] = o[2];
+                } else {
+                    const uint16_t* o = (const uint16_t*)data + i * 3;
+                    w[0] = (uint32_t)o[0] | ((uint32_t)o[1] << 16);
+                    w[1] = (w[1] & 0xFFFF0000u) | o[2];
+                }
+            } else if (col) {
+                w[0] = (w[0] & 0xFFFFu) | ((uint32_t)((const uint8_t*)data)[i] << 16);
+            } else {
+                w[0] = (w[0] & 0xFFFF0000u) | ((const uint16_t*)data)[i];
+            }
+        }
+        return LVBGS_OK;
+    }
+    struct Item {
+        const char* name;
+        void* ptr;
+        size_t bytes;
+    };
+    const size_t fb = npx * 4, nds = (size_t)v.dsW * v.dsH * C * 4;
+    const Item items[] = {{"last_color", v.last_color, npx * C},  {"last_desc", v.last_desc, npx * C * 2},
+                          {"last_fg", v.last_fg, npx},             {"T", v.T, fb},
+                          {"R", v.R, fb},                          {"V", v.V, fb},
+                          {"MLD", v.MLD, fb},                      {"DminLT", v.DminLT, fb},
+                          {"DminST", v.DminST, fb},                {"RawLT", v.RawLT, fb},
+                          {"RawST", v.RawST, fb},                  {"FinLT", v.FinLT, fb},
+                          {"FinST", v.FinST, fb},                  {"unstable", v.unstable, npx},
+                          {"blinks", v.blinks, npx},               {"last_raw", v.last_raw, npx},
+                          {"last_raw_blink", v.last_raw_blink, npx}, {"last_dil_inv", v.last_dil_inv, npx},
+                          {"dsLT", v.dsLT, nds},                   {"dsST", v.dsST, nds}};
+    for (const auto& it : items)
+        if (n == it.name) {
+            if (!it.ptr || it.bytes != bytes) return set_err(c, LVBGS_ERR_ARG, (std::string("buffer size mismatch: ") + name).c_str());
+            return hip_check(c, hipMemcpy(it.ptr, data, bytes, hipMemcpyHostToDevice), name);
+        }
+    return set_err(c, LVBGS_ERR_ARG, (std::string("unknown buffer ") + name).c_str());
+}
+
+int lvbgs_restore_scalars(lvbgs_ctx* c, const Scalars& s, uint32_t apply_count) {
+    std::vector<uint32_t>& r = restore_records(c);
+    int rc = hip_check(c, hipMemcpy(c->v.sA, r.data(), r.size() * 4, hipMemcpyHostToDevice), "samples");
+    drop_restore_records(c);
+    if (rc) return rc;
+    if ((rc = hip_check(c, hipMemcpy(c->v.sc, &s, sizeof(Scalars), hipMemcpyHostToDevice), "scalars"))) return rc;
+    c->apply_count = apply_count;
+    c->initialized = true;
+    return LVBGS_OK;
 }
 
 extern "C" int lvbgs_set_timing(lvbgs_ctx* c, const char* stage) {

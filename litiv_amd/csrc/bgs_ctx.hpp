@@ -27,6 +27,8 @@ struct lvbgs_ctx {
     lvbgs::View v{};
     uint8_t* d_frame = nullptr;  // staging for host-pointer apply / initialize
     uint8_t* d_mask = nullptr;
+    int ring_frames = 0;         // serving: device slots for frames submitted from host memory
+    uint8_t* d_ring = nullptr;   // ring_frames frames of W*H*C bytes (in the arena)
     lvbgs::StepOptions opt{};
     uint32_t apply_count = 0;
     // timing
@@ -36,9 +38,13 @@ struct lvbgs_ctx {
 };
 
 
-// device bytes of the model and per-step buffers of a stream of this geometry (one arena)
-size_t lvbgs_arena_bytes(int algo, const lvbgs_params& p, int width, int height, int channels);
+// device bytes of the model and per-step buffers of a stream of this geometry (one arena), with
+// ring_frames extra frame slots for host-memory ingest
+size_t lvbgs_arena_bytes(int algo, const lvbgs_params& p, int width, int height, int channels, int ring_frames = 0);
 
 // initialize from a device frame with the previous full-frame ROI of the same size (or a full frame),
 // left queued on the context's stream; `done` is recorded after it
 int lvbgs_initialize_async(lvbgs_ctx* c, const void* d_img, int width, int height, int channels, hipEvent_t done);
+// the same from a frame in host memory (copied to the context's staging frame on its stream)
+int lvbgs_initialize_async_host(lvbgs_ctx* c, const void* img, int width, int height, int channels, size_t stride,
+                                hipEvent_t done);

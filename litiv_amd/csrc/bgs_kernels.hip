@@ -45,7 +45,9 @@ LV_D float fmin_ref(float a, float b) { return (b < a) ? b : a; }  // std::min
 LV_D float fmax_ref(float a, float b) { return (a < b) ? b : a; }  // std::max
 
 // Workgroup sum of one value per thread, added to stripe blockIdx % kStripes of a counter array.
-// Every thread of the workgroup must call it.
+// Every thread of the workgroup must call it. The closing barrier keeps the wavefront partials in
+// LDS until thread 0 has summed them, so a second call right after (the model update adds two
+// counters) cannot overwrite them early.
 template <int NT>
 LV_D void block_add(unsigned long long value, unsigned long long* stripes) {
     __shared__ unsigned long long part[NT / 64];
@@ -59,6 +61,7 @@ LV_D void block_add(unsigned long long value, unsigned long long* stripes) {
         for (int i = 0; i < NT / 64; ++i) s += part[i];
         if (s) atomicAdd(&stripes[blockIdx.x % kStripes], s);
     }
+    __syncthreads();
 }
 
 LV_HD unsigned long long sum_stripes(const unsigned long long* s) {
@@ -295,12 +298,25 @@ LV_D bool test_sample(const PxCtx<C>& px, const uint8_t* lut, const uint32_t col
             cdv[c] = absdiff(px.cur[c], col[c]);
             if (cdv[c] > px.t_sc) return false;
         }
+        // lower bounds from the intra distances alone (the inter distance is >= 0): a sample they
+        // already fail needs no inter descriptor
+        uint32_t hd[3], lb_d = 0, lb_s = 0;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            hd[c] = popc(px.intra[c] ^ desc[c]);
+            uint32_t s = ((hd[c] >> 1) >> 1) * 15u + cdv[c];
+            s = s > 255u ? 255u : s;
+            if (s > px.t_sc) return false;
+            lb_d += hd[c] >> 1;
+            lb_s += s;
+        }
+        if (lb_d > px.t_tot_desc || lb_s > px.t_tot_col) return false;
         uint32_t td = 0, ts = 0;
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             const uint32_t cd = cdv[c];
             const uint32_t inter = lbsp_threshold_packed(px.taps[c], col[c], lut[col[c]]);
-            const uint32_t dd = (popc(px.intra[c] ^ desc[c]) + popc(inter ^ desc[c])) >> 1;
+            const uint32_t dd = (hd[c] + popc(inter ^ desc[c])) >> 1;
             uint32_t sum = (dd >> 1) * 15u + cd;
             sum = sum > 255u ? 255u : sum;
             if (sum > px.t_sc) return false;
@@ -314,8 +330,13 @@ LV_D bool test_sample(const PxCtx<C>& px, const uint8_t* lut, const uint32_t col
     } else if constexpr (ALGO == kSuBSENSE && C == 1) {
         const uint32_t cd = absdiff(px.cur[0], col[0]);
         if (cd > px.t_col) return false;
+        // lower bounds from the intra distance alone (the inter distance is >= 0)
+        const uint32_t hd = popc(px.intra[0] ^ desc[0]);
+        if ((hd >> 1) > px.t_desc) return false;
+        uint32_t lb = ((hd >> 1) >> 2) * 15u + cd;
+        if ((lb > 255u ? 255u : lb) > px.t_col) return false;
         const uint32_t inter = lbsp_threshold_packed(px.taps[0], col[0], lut[col[0]]);
-        const uint32_t dd = (popc(px.intra[0] ^ desc[0]) + popc(inter ^ desc[0])) >> 1;
+        const uint32_t dd = (hd + popc(inter ^ desc[0])) >> 1;
         if (dd > px.t_desc) return false;
         uint32_t sum = (dd >> 2) * 15u + cd;
         sum = sum > 255u ? 255u : sum;
@@ -684,7 +705,7 @@ LV_D void describe_body(const View& v, uint32_t bid, uint32_t nblk) {
             auto diff_slot = [&]() {
                 uint32_t d4[4];
                 draw4(v.seed_lo, v.seed_hi, v.stream_id, frame_no, PH_APPLY, 0, p, 1, d4);
-                return d4[0] % N;
+                return mod64(d4[0], N);
             };
             uint32_t r;
             if constexpr (ALGO == kSuBSENSE) {
@@ -707,7 +728,7 @@ LV_D void describe_body(const View& v, uint32_t bid, uint32_t nblk) {
                 const uint32_t tg = (uint32_t)((ty - y + 2) * 5 + (tx - x + 2));
                 const uint32_t sd = (gate_a || gate_b) ? diff_slot() : 0u;
                 r = tg | ((uint32_t)gate_a << 5) | ((uint32_t)gate_b << 6) | ((uint32_t)ghost_fwd << 7) |
-                    ((uint32_t)self_bg << 8) | ((uint32_t)self_fg << 9) | (unst_new << 10) | ((d[1] % N) << 16) |
+                    ((uint32_t)self_bg << 8) | ((uint32_t)self_fg << 9) | (unst_new << 10) | (mod64(d[1], N) << 16) |
                     (sd << 24);
             } else {
                 const uint64_t L = isinf(lr) ? ~0ull : (uint64_t)ceil(lr);
@@ -717,7 +738,7 @@ LV_D void describe_body(const View& v, uint32_t bid, uint32_t nblk) {
                 neighbor_pos(d[3], true, x, y, v.W, v.H, tx, ty);
                 const uint32_t tg = (uint32_t)((ty - y + 2) * 5 + (tx - x + 2));
                 const uint32_t sd = gate_a ? diff_slot() : 0u;
-                r = tg | ((uint32_t)gate_a << 5) | ((uint32_t)self_bg << 8) | ((d[1] % N) << 16) | (sd << 24);
+                r = tg | ((uint32_t)gate_a << 5) | ((uint32_t)self_bg << 8) | (mod64(d[1], N) << 16) | (sd << 24);
             }
             v.rec[p] = r;
         }

@@ -14,6 +14,12 @@
 // completed with LVBGS_ERR_DEVICE and is closed by the server; its id stays reserved until
 // lvbgs_server_close acknowledges the failure, then the context goes to the pool (re-initialization
 // clears the bits). The other streams of the step are unaffected.
+//
+// Host ingest (options.host_ring > 0): every stream's arena holds host_ring frame slots. A frame
+// submitted from host memory is copied at once into the stream's next free slot on the server's
+// copy stream, with an event after it, so the copy overlaps the steps already running; the step
+// that takes the frame waits for that event on the device. Slots are freed in order as the
+// stream's frames complete; a stream with every slot busy refuses more (LVBGS_ERR_BUSY).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -35,6 +41,7 @@ struct Request {
     uint8_t* mask;
     double lr;
     uint64_t tag;
+    int ring_slot;  // host-ingest slot holding img (-1: a caller's device frame)
 };
 
 struct Slot {
@@ -45,12 +52,15 @@ struct Slot {
     bool closing = false;
     bool failed = false;  // a step reported a device error for this stream: closed by the server
     bool ready = false;  // its initialization (queued on the context's stream) has completed
+    int ring_next = 0;   // next host-ingest slot to fill
+    int ring_busy = 0;   // host-ingest slots holding frames not completed yet
 };
 
 struct StepFrame {
     int stream;
     uint64_t tag;
-    int pos;  // the stream's View slot in the step (index into the gathered error bits)
+    int pos;        // the stream's View slot in the step (index into the gathered error bits)
+    int ring_slot;  // host-ingest slot of the frame (-1: none)
 };
 
 struct QueuedStep {
@@ -70,6 +80,8 @@ struct lvbgs_server {
     size_t used = 0;  // arena bytes of open and pooled contexts
     std::vector<Slot> slots;
     std::vector<hipEvent_t> init_ev;  // per slot id: end of the stream's initialization
+    std::vector<std::vector<hipEvent_t>> ring_ev;  // per slot id: end of each host-ingest copy
+    hipStream_t cstream = nullptr;                 // host-ingest copies
     std::vector<int> free_ids;
     std::vector<lvbgs_ctx*> pool;  // closed contexts whose arenas are kept for reuse
     View* d_views = nullptr;
@@ -112,6 +124,7 @@ extern "C" int lvbgs_server_default_options(lvbgs_server_options* o) {
     o->hbm_budget = (size_t)250 << 30;
     o->max_inflight = 2;
     o->max_batch = 256;
+    o->host_ring = 0;
     lvbgs_batch_default_options(&o->launch);
     o->launch.use_graph = 0;
     return LVBGS_OK;
@@ -131,6 +144,9 @@ static void server_release(lvbgs_server* s) {
     for (auto e : s->ev_free) (void)hipEventDestroy(e);
     for (auto& q : s->queued) (void)hipEventDestroy(q.done);
     for (auto e : s->init_ev) (void)hipEventDestroy(e);
+    for (auto& v : s->ring_ev)
+        for (auto e : v) (void)hipEventDestroy(e);
+    if (s->cstream) (void)hipStreamDestroy(s->cstream);
     for (int g = 1; g < lvbgs_server::kMaxGroups; ++g)
         if (s->gstream[g]) (void)hipStreamDestroy(s->gstream[g]);
     for (auto e : s->ev_join)
@@ -149,7 +165,8 @@ extern "C" int lvbgs_server_create(int device, const lvbgs_server_options* opts,
     if (o.max_inflight < 1 || o.max_inflight > 16 || o.max_batch < 1 || o.max_batch > 4096 || L.seq_limit < 1 ||
         L.seq_limit > 255 || L.cont_blocks < 1 || L.fixpoint_launches < 0 || L.fixpoint_launches > 64 ||
         L.fixpoint_blocks < 1 || L.cont_px_per_block < 0 || L.fix_px_per_block < 0 ||
-        (L.fetch_depth != 2 && L.fetch_depth != 4) || L.groups < 1 || L.groups > lvbgs_server::kMaxGroups)
+        (L.fetch_depth != 2 && L.fetch_depth != 4) || L.groups < 1 || L.groups > lvbgs_server::kMaxGroups ||
+        o.host_ring < 0 || o.host_ring > 64)
         return LVBGS_ERR_ARG;
     if (hipSetDevice(device) != hipSuccess) return LVBGS_ERR_HIP;
     lvbgs_server* s = new lvbgs_server();
@@ -172,6 +189,7 @@ extern "C" int lvbgs_server_create(int device, const lvbgs_server_options* opts,
         ok = hipStreamCreateWithFlags(&s->gstream[g], hipStreamNonBlocking) == hipSuccess &&
              hipEventCreateWithFlags(&s->ev_join[g], hipEventDisableTiming) == hipSuccess;
     if (ok && s->groups > 1) ok = hipEventCreateWithFlags(&s->ev_fork, hipEventDisableTiming) == hipSuccess;
+    if (ok && o.host_ring > 0) ok = hipStreamCreateWithFlags(&s->cstream, hipStreamNonBlocking) == hipSuccess;
     const int nstage = o.max_inflight + 1;
     for (int i = 0; ok && i < nstage; ++i) {
         uint8_t* p = nullptr;
@@ -216,7 +234,10 @@ static void fail_stream(lvbgs_server* s, int id) {
     if (sl.failed) return;
     sl.failed = true;
     sl.closing = true;
-    for (const Request& r : sl.pending) s->done.push_back(lvbgs_done{r.tag, id, LVBGS_ERR_DEVICE});
+    for (const Request& r : sl.pending) {
+        s->done.push_back(lvbgs_done{r.tag, id, LVBGS_ERR_DEVICE});
+        if (r.ring_slot >= 0) --sl.ring_busy;
+    }
     s->st.frames_done += sl.pending.size();
     sl.pending.clear();
 }
@@ -234,6 +255,7 @@ static int harvest(lvbgs_server* s, bool wait_one) {
             const bool bad = (q.h_err[f.pos] & kErrFixpointCap) != 0 || sl.failed;
             s->done.push_back(lvbgs_done{f.tag, f.stream, bad ? LVBGS_ERR_DEVICE : LVBGS_OK});
             --sl.inflight;
+            if (f.ring_slot >= 0) --sl.ring_busy;
             if (bad) fail_stream(s, f.stream);
         }
         s->st.frames_done += q.frames.size();
@@ -285,7 +307,12 @@ static int launch_step(lvbgs_server* s) {
         sl.pending.pop_front();
         ++sl.inflight;
         ++sl.ctx->apply_count;
-        q.frames.push_back(StepFrame{members[(size_t)m], req[(size_t)m].tag, 0});
+        q.frames.push_back(StepFrame{members[(size_t)m], req[(size_t)m].tag, 0, req[(size_t)m].ring_slot});
+        // a frame copied from host memory: the step waits for its copy
+        if (req[(size_t)m].ring_slot >= 0 &&
+            hipStreamWaitEvent(s->stream, s->ring_ev[(size_t)members[(size_t)m]][(size_t)req[(size_t)m].ring_slot],
+                               0) != hipSuccess)
+            return srv_err(s, LVBGS_ERR_HIP, "ingest wait");
     }
     // groups: largest frames first, dealt round-robin; a group is a contiguous range of View slots
     const int G = std::min(s->groups, n);
@@ -406,14 +433,15 @@ extern "C" int lvbgs_server_drain(lvbgs_server* s) {
     return harvest(s, false);
 }
 
-extern "C" int lvbgs_server_open(lvbgs_server* s, const lvbgs_params* params, const void* d_img0, int width,
-                                 int height, int channels, int* stream_out) {
-    if (!s || !params || !d_img0 || !stream_out || width <= 0 || height <= 0 || (channels != 1 && channels != 3))
+// admits a stream initialized from img0: a device frame, or host memory with row stride host_stride
+static int server_open(lvbgs_server* s, const lvbgs_params* params, const void* img0, bool host, size_t host_stride,
+                       int width, int height, int channels, int* stream_out) {
+    if (!s || !params || !img0 || !stream_out || width <= 0 || height <= 0 || (channels != 1 && channels != 3))
         return s ? srv_err(s, LVBGS_ERR_ARG, "invalid stream parameters") : LVBGS_ERR_ARG;
     (void)hipSetDevice(s->device);
     lvbgs_params p = *params;
     p.device = s->device;
-    const size_t need = lvbgs_arena_bytes(LVBGS_SUBSENSE, p, width, height, channels);
+    const size_t need = lvbgs_arena_bytes(LVBGS_SUBSENSE, p, width, height, channels, s->o.host_ring);
     // a pooled context of the same geometry and model parameters keeps its arena (no allocation)
     lvbgs_ctx* c = nullptr;
     for (size_t i = 0; i < s->pool.size(); ++i)
@@ -432,6 +460,7 @@ extern "C" int lvbgs_server_open(lvbgs_server* s, const lvbgs_params* params, co
         if (s->used + need > s->o.hbm_budget) return srv_err(s, LVBGS_ERR_BUDGET, "model does not fit the budget");
         const int rc = lvbgs_create(LVBGS_SUBSENSE, &p, &c);
         if (rc) return srv_err(s, rc, "invalid stream parameters");
+        c->ring_frames = s->o.host_ring;
     } else {
         c->p.rng_seed = p.rng_seed;
         c->p.stream_id = p.stream_id;
@@ -447,14 +476,28 @@ extern "C" int lvbgs_server_open(lvbgs_server* s, const lvbgs_params* params, co
             else lvbgs_destroy(c);
             return srv_err(s, LVBGS_ERR_HIP, "event");
         }
+        std::vector<hipEvent_t> rev((size_t)s->o.host_ring, nullptr);
+        bool ok = true;
+        for (auto& x : rev) ok = ok && hipEventCreateWithFlags(&x, hipEventDisableTiming) == hipSuccess;
+        if (!ok) {
+            for (auto x : rev)
+                if (x) (void)hipEventDestroy(x);
+            (void)hipEventDestroy(e);
+            if (c->arena_size) s->pool.push_back(c);
+            else lvbgs_destroy(c);
+            return srv_err(s, LVBGS_ERR_HIP, "event");
+        }
         id = (int)s->slots.size();
         s->slots.emplace_back();
         s->init_ev.push_back(e);
+        s->ring_ev.push_back(std::move(rev));
     }
     // initialization is queued on the context's own stream (no host wait): the stream joins steps
     // once its event has completed
     const size_t before = c->arena_size;
-    const int rc = lvbgs_initialize_async(c, d_img0, width, height, channels, s->init_ev[(size_t)id]);
+    const int rc = host ? lvbgs_initialize_async_host(c, img0, width, height, channels, host_stride,
+                                                      s->init_ev[(size_t)id])
+                        : lvbgs_initialize_async(c, img0, width, height, channels, s->init_ev[(size_t)id]);
     if (rc) {
         s->err = lvbgs_last_error(c);
         s->used -= before;
@@ -469,6 +512,16 @@ extern "C" int lvbgs_server_open(lvbgs_server* s, const lvbgs_params* params, co
     sl.open = true;
     *stream_out = id;
     return LVBGS_OK;
+}
+
+extern "C" int lvbgs_server_open(lvbgs_server* s, const lvbgs_params* params, const void* d_img0, int width,
+                                 int height, int channels, int* stream_out) {
+    return server_open(s, params, d_img0, false, 0, width, height, channels, stream_out);
+}
+
+extern "C" int lvbgs_server_open_host(lvbgs_server* s, const lvbgs_params* params, const void* img0, size_t stride,
+                                      int width, int height, int channels, int* stream_out) {
+    return server_open(s, params, img0, true, stride, width, height, channels, stream_out);
 }
 
 extern "C" int lvbgs_server_close(lvbgs_server* s, int stream) {
@@ -491,9 +544,56 @@ extern "C" int lvbgs_server_submit(lvbgs_server* s, int stream, const void* d_im
     if (!s || stream < 0 || stream >= (int)s->slots.size() || !s->slots[(size_t)stream].open ||
         s->slots[(size_t)stream].closing || !d_img)
         return s ? srv_err(s, LVBGS_ERR_ARG, "unknown or closing stream, or null frame") : LVBGS_ERR_ARG;
-    s->slots[(size_t)stream].pending.push_back(Request{(const uint8_t*)d_img, (uint8_t*)d_mask, lr, tag});
+    s->slots[(size_t)stream].pending.push_back(Request{(const uint8_t*)d_img, (uint8_t*)d_mask, lr, tag, -1});
     ++s->st.frames_submitted;
     return LVBGS_OK;
+}
+
+extern "C" int lvbgs_server_submit_host(lvbgs_server* s, int stream, const void* host_img, size_t stride,
+                                        void* d_mask, double lr, uint64_t tag) {
+    if (s && stream >= 0 && stream < (int)s->slots.size() && s->slots[(size_t)stream].failed)
+        return srv_err(s, LVBGS_ERR_DEVICE, "stream closed after a device error");
+    if (!s || stream < 0 || stream >= (int)s->slots.size() || !s->slots[(size_t)stream].open ||
+        s->slots[(size_t)stream].closing || !host_img)
+        return s ? srv_err(s, LVBGS_ERR_ARG, "unknown or closing stream, or null frame") : LVBGS_ERR_ARG;
+    if (s->o.host_ring <= 0) return srv_err(s, LVBGS_ERR_ARG, "the server has no host-ingest slots (host_ring 0)");
+    Slot& sl = s->slots[(size_t)stream];
+    if (sl.ring_busy >= s->o.host_ring) return srv_err(s, LVBGS_ERR_BUSY, "every host-ingest slot of the stream is busy");
+    lvbgs_ctx* c = sl.ctx;
+    const size_t row = (size_t)c->W * (size_t)c->C;
+    if (stride == 0) stride = row;
+    if (stride < row) return srv_err(s, LVBGS_ERR_ARG, "stride too small");
+    (void)hipSetDevice(s->device);
+    const int k = sl.ring_next;
+    uint8_t* d = c->d_ring + (size_t)k * row * (size_t)c->H;
+    // the slot's previous frame has completed (slots are freed in order as frames complete)
+    if (hipMemcpy2DAsync(d, row, host_img, stride, row, (size_t)c->H, hipMemcpyHostToDevice, s->cstream) !=
+            hipSuccess ||
+        hipEventRecord(s->ring_ev[(size_t)stream][(size_t)k], s->cstream) != hipSuccess)
+        return srv_err(s, LVBGS_ERR_HIP, "ingest copy");
+    sl.ring_next = (k + 1) % s->o.host_ring;
+    ++sl.ring_busy;
+    sl.pending.push_back(Request{d, (uint8_t*)d_mask, lr, tag, k});
+    ++s->st.frames_submitted;
+    return LVBGS_OK;
+}
+
+extern "C" size_t lvbgs_server_stream_bytes(const lvbgs_server* s, const lvbgs_params* params, int width, int height,
+                                            int channels) {
+    lvbgs_params d;
+    if (!s || lvbgs_default_params(LVBGS_SUBSENSE, &d) != LVBGS_OK || width <= 0 || height <= 0 ||
+        (channels != 1 && channels != 3))
+        return 0;
+    return lvbgs_arena_bytes(LVBGS_SUBSENSE, params ? *params : d, width, height, channels, s->o.host_ring);
+}
+
+extern "C" void* lvbgs_host_alloc(size_t bytes) {
+    void* p = nullptr;
+    return hipHostMalloc(&p, bytes) == hipSuccess ? p : nullptr;
+}
+
+extern "C" void lvbgs_host_free(void* p) {
+    if (p) (void)hipHostFree(p);
 }
 
 extern "C" lvbgs_ctx* lvbgs_server_context(lvbgs_server* s, int stream) {

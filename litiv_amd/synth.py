@@ -4,6 +4,8 @@ import os
 
 import numpy as np
 
+from . import _native
+
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "liblitiv_synth.so")
 
@@ -34,7 +36,7 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise RuntimeError("liblitiv_synth.so is not built: run __graft_entry__.build()")
-        L = C.CDLL(LIB_PATH)
+        L = _native.load(LIB_PATH)
         L.lvsynth_preset.argtypes = [C.c_int, C.c_int, C.POINTER(Config)]
         L.lvsynth_build_scene.argtypes = [C.POINTER(Config), C.POINTER(Scene)]
         L.lvsynth_frame_host.argtypes = [C.POINTER(Scene), C.c_int, C.c_void_p]

@@ -230,8 +230,9 @@ def test_serving_full_size_resolutions_with_admission_churn():
 
 def test_device_error_is_reported_for_one_stream_only():
     """The fix-point cap set to 0 from the host (no full-grid fix-point launches): a stream whose raster-
-    order fix point has work stops with LVBGS_ERR_DEVICE (every kernel exits normally); a static
-    stream (no decision ever changes, no fix-point work) in the same steps keeps its exact results.
+    order fix point has work stops with LVBGS_ERR_DEVICE (every kernel exits normally); a flat, static
+    stream in the same steps (every sample equals the frame, every pixel is background in every frame,
+    so no decision ever differs from its guess and the fix point has no work) keeps its exact results.
     The failed stream's later frames complete with the same status, further submits are refused, and
     after close (which acknowledges the failure) its pooled context serves a new stream exactly."""
     from litiv_amd.bgs import ERR_DEVICE, LvbgsError
@@ -239,9 +240,10 @@ def test_device_error_is_reported_for_one_stream_only():
     n = 10
     moving = Stream(torch, 0, 96, 72, 3, n)
     still = Stream(torch, 1, 80, 64, 3, n)
-    f0 = still.frames[0]
+    f0 = np.empty((64, 80, 3), np.uint8)
+    f0[...] = (90, 120, 150)
     still.frames = [f0] * n
-    still.d = [still.d[0]] * n
+    still.d = [torch.from_numpy(f0).cuda()] * n
     o = Oracle(SUBSENSE, **still.params)
     o.initialize(f0)
     still.expected = [o.apply(f0) for _ in range(n)]
@@ -265,6 +267,7 @@ def test_device_error_is_reported_for_one_stream_only():
         assert np.array_equal(moving.m[t].cpu().numpy(), moving.expected[t]), t
     for t in range(n):
         assert np.array_equal(still.m[t].cpu().numpy(), still.expected[t]), t
+        assert not still.expected[t].any()
     with pytest.raises(LvbgsError) as e:
         srv.submit(sm, moving.d[0].data_ptr(), moving.m[0].data_ptr())
     assert e.value.code == ERR_DEVICE
@@ -284,3 +287,50 @@ def test_device_error_is_reported_for_one_stream_only():
     for t in range(n):
         assert np.array_equal(again.m[t].cpu().numpy(), again.expected[t]), t
     srv.close_server()
+
+
+def test_host_ingest_matches_device_frames():
+    """Frames submitted from pinned host memory (lvbgs_server_submit_host: copied at once into the
+    stream's ingest slots on the copy stream, waited for on the device by the step that takes them)
+    give the oracle's masks, mixed in the same steps with a stream fed device frames; a stream whose
+    slots are all busy refuses more until frames complete."""
+    from litiv_amd.bgs import PinnedBuffer
+    torch = _torch()
+    n = 12
+    hs = Stream(torch, 5, 128, 96, 3, n)       # host-fed
+    ds = Stream(torch, 6, 96, 72, 3, n)        # device-fed
+    torch.cuda.synchronize()
+    ring = 3
+    srv = Server(hbm_budget=1 << 30, max_inflight=2, host_ring=ring)
+    assert srv.stream_bytes(128, 96, 3) >= Server.model_bytes(128, 96, 3) + ring * 128 * 96 * 3
+    pinned = PinnedBuffer(n * hs.frames[0].nbytes)
+    fb = hs.frames[0].nbytes
+    for t in range(n):
+        pinned.array[t * fb:(t + 1) * fb] = hs.frames[t].reshape(-1)
+    sh = srv.open_host(hs.frames[0], **hs.params)
+    sd = _open(srv, ds)
+    statuses, t_h, t_d = {}, 0, 0
+    # no step runs before a pump: the ring fills and the next host frame is refused
+    for t_h in range(ring):
+        assert srv.submit_host(sh, pinned.ptr + t_h * fb, hs.m[t_h].data_ptr(), 0.0, tag=t_h)
+    t_h = ring
+    assert not srv.submit_host(sh, pinned.ptr + t_h * fb, hs.m[t_h].data_ptr(), 0.0, tag=t_h)
+    while t_h < n or t_d < n:
+        if t_h < n and srv.submit_host(sh, pinned.ptr + t_h * fb, hs.m[t_h].data_ptr(), 0.0, tag=t_h):
+            t_h += 1
+        if t_d < n:
+            srv.submit(sd, ds.d[t_d].data_ptr(), ds.m[t_d].data_ptr(), 0.0, tag=100 + t_d)
+            t_d += 1
+        srv.pump()
+        for tag, sid, st in srv.poll():
+            statuses[int(tag)] = int(st)
+    srv.drain()
+    for tag, sid, st in srv.poll():
+        statuses[int(tag)] = int(st)
+    assert len(statuses) == 2 * n and all(v == 0 for v in statuses.values())
+    torch.cuda.synchronize()
+    for t in range(n):
+        assert np.array_equal(hs.m[t].cpu().numpy(), hs.expected[t]), "host-fed frame %d" % t
+        assert np.array_equal(ds.m[t].cpu().numpy(), ds.expected[t]), "device-fed frame %d" % t
+    srv.close_server()
+    pinned.free()
