@@ -27,6 +27,7 @@ extern "C" {
 #define LVBGS_ERR_DEVICE (-5)   /* device-side consistency check failed */
 #define LVBGS_ERR_BUDGET (-6)   /* a server's device-memory budget cannot hold the model */
 #define LVBGS_ERR_BUSY (-7)     /* a served stream's host-ingest slots are all busy: poll, then retry */
+#define LVBGS_ERR_TOO_LARGE (-8) /* input beyond a fixed-capacity entry point's stated maximum */
 
 enum { LVBGS_SUBSENSE = 1, LVBGS_LOBSTER = 2 };
 
@@ -305,6 +306,16 @@ typedef struct lvbgs_binclassif {
  * buffer per call (about 8 bytes per 1024 pixels). */
 int lvbgs_binclassif_accumulate(const void* d_classif, const void* d_gt, const void* d_roi, int width, int height,
                                 size_t stride, lvbgs_binclassif* counts, void* stream);
+/* The same counts without any device allocation. No counterpart in the reference: it is for callers
+ * that cannot allocate (code under HIP graph capture, serving loops and harnesses that forbid
+ * hipMalloc). The partial counts live in a static 512 KiB device array of the library, one copy per
+ * device that uses it, held until the process ends; so a frame may hold at most
+ * LVBGS_BINCLASSIF_NOALLOC_MAX_PIXELS pixels (LVBGS_ERR_TOO_LARGE beyond, before any launch). Every
+ * call on one device shares that array: calls that overlap on one device (from several host threads)
+ * race and must be serialized by the caller. Arguments and results as lvbgs_binclassif_accumulate. */
+#define LVBGS_BINCLASSIF_NOALLOC_MAX_PIXELS 67100672ull
+int lvbgs_binclassif_accumulate_noalloc(const void* d_classif, const void* d_gt, const void* d_roi, int width,
+                                        int height, size_t stride, lvbgs_binclassif* counts, void* stream);
 
 /* ---- binary matrix archives and checkpoint / resume ----
  * lv::write / lv::read with lv::MatArchive_BINARY (modules/utils/src/opencv.cpp:514-531, :608-625):
@@ -319,7 +330,7 @@ long long lvbgs_archive_read(const char* path, int* cv_type, int* dims, int32_t*
                              size_t cap);
 /* Writes the whole model of an initialized context into the existing directory dir, one archive per
  * state buffer (the getters' layouts) plus meta.bin and lut.bin. The reference keeps its model only
- * in memory (BackgroundSubtractorSuBSENSE.hpp:70-128); a checkpoint lets a stream resume after a
+ * in memory (BackgroundSubtractorSuBSENSE.hpp:71-131); a checkpoint lets a stream resume after a
  * restart. */
 int lvbgs_checkpoint_save(lvbgs_ctx* ctx, const char* dir);
 /* Restores a checkpoint into a context created with the same algorithm, parameters, rng_seed and

@@ -83,6 +83,9 @@ class ServerStats(C.Structure):
 DONE_DTYPE = np.dtype([("tag", np.uint64), ("stream", np.int32), ("status", np.int32)])
 ERR_BUDGET = -6
 ERR_BUSY = -7
+ERR_TOO_LARGE = -8
+# include/litiv_bgs.h LVBGS_BINCLASSIF_NOALLOC_MAX_PIXELS: the no-alloc evaluator's frame limit
+BINCLASSIF_NOALLOC_MAX_PIXELS = 67100672
 
 _lib = None
 
@@ -159,6 +162,8 @@ def lib():
         L.lvbgs_server_context.restype = vp
         L.lvbgs_server_debug_fixpoint_cap.argtypes = [vp, i]
         L.lvbgs_binclassif_accumulate.argtypes = [vp, vp, vp, i, i, C.c_size_t, C.POINTER(BinClassifCounts), vp]
+        L.lvbgs_binclassif_accumulate_noalloc.argtypes = [vp, vp, vp, i, i, C.c_size_t, C.POINTER(BinClassifCounts),
+                                                          vp]
         L.lvbgs_archive_write.argtypes = [C.c_char_p, i, i, C.POINTER(C.c_int32), vp]
         L.lvbgs_archive_read.argtypes = [C.c_char_p, C.POINTER(i), C.POINTER(i), C.POINTER(C.c_int32), i, vp,
                                          C.c_size_t]

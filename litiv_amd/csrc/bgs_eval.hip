@@ -12,10 +12,12 @@
 // Two kernels: each block counts a chunk of kChunk pixels (16 per thread, 16-byte loads when the
 // rows are packed and the pointers aligned) into one partial record; one block then sums the
 // partials in a fixed order into six 64-bit totals. The partials and totals live in a scratch
-// buffer allocated per call. Every partial record and total is written before it is read, so the
-// scratch needs no clearing.
+// buffer: allocated per call by lvbgs_binclassif_accumulate, or the file-scope `scratch` array (one
+// copy per device) for lvbgs_binclassif_accumulate_noalloc. Every partial record and total is
+// written before it is read, so the scratch needs no clearing between calls.
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdint>
 #include <cstring>
 
@@ -29,6 +31,18 @@ constexpr uint32_t kChunk = kThreads * kPerThread;  // pixels per block
 constexpr int kCounters = 6;                         // TP, TN, FP, FN, SE, DC
 constexpr size_t kPartialBytes = 32;                 // one block's partial record (6 u32, padded)
 constexpr size_t kTotalsBytes = 64;                  // the final totals (6 u64, padded)
+
+// the no-alloc entry point's scratch: partial records of up to kMaxBlocks blocks, then the totals
+constexpr size_t kScratchBytes = (size_t)512 << 10;
+constexpr uint32_t kMaxBlocks = (uint32_t)((kScratchBytes - kTotalsBytes) / kPartialBytes);
+static_assert((uint64_t)kMaxBlocks * kChunk == LVBGS_BINCLASSIF_NOALLOC_MAX_PIXELS,
+              "the header's pixel limit is the scratch's capacity");
+
+}  // namespace
+
+static __device__ __align__(256) uint8_t scratch[kScratchBytes];
+
+namespace {
 
 struct Geometry {
     const uint8_t* classif;
@@ -155,6 +169,19 @@ int run(const void* d_classif, const void* d_gt, const void* d_roi, int width, i
     return LVBGS_OK;
 }
 
+// per device: the device address of `scratch`, taken once with that device current (in host code
+// the array's name is not a device address)
+constexpr int kMaxDevices = 64;
+std::atomic<void*> g_scratch_addr[kMaxDevices];
+
+void* scratch_address(int dev) {
+    void* p = g_scratch_addr[dev].load(std::memory_order_acquire);
+    if (p) return p;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return nullptr;
+    g_scratch_addr[dev].store(p, std::memory_order_release);  // a racing first call stores the same value
+    return p;
+}
+
 }  // namespace
 
 extern "C" int lvbgs_binclassif_accumulate(const void* d_classif, const void* d_gt, const void* d_roi, int width,
@@ -171,4 +198,22 @@ extern "C" int lvbgs_binclassif_accumulate(const void* d_classif, const void* d_
     rc = run(d_classif, d_gt, d_roi, width, height, stride, counts, (hipStream_t)stream, buf, nblk);
     if (hipFree(buf) != hipSuccess && rc == LVBGS_OK) rc = LVBGS_ERR_HIP;
     return rc;
+}
+
+extern "C" int lvbgs_binclassif_accumulate_noalloc(const void* d_classif, const void* d_gt, const void* d_roi,
+                                                   int width, int height, size_t stride, lvbgs_binclassif* counts,
+                                                   void* stream) {
+    uint32_t nblk = 0;
+    int rc = check_args(d_classif, width, height, stride, counts, &nblk);
+    if (rc) return rc;
+    if ((uint64_t)width * (uint64_t)height > LVBGS_BINCLASSIF_NOALLOC_MAX_PIXELS) return LVBGS_ERR_TOO_LARGE;
+    if (!d_gt) {
+        counts->dc += (uint64_t)width * (uint64_t)height;
+        return LVBGS_OK;
+    }
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return LVBGS_ERR_HIP;
+    uint8_t* buf = static_cast<uint8_t*>(scratch_address(dev));
+    if (!buf) return LVBGS_ERR_HIP;
+    return run(d_classif, d_gt, d_roi, width, height, stride, counts, (hipStream_t)stream, buf, nblk);
 }

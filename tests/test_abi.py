@@ -108,3 +108,12 @@ def test_cpp_mirror_compiles_and_links(tmp_path):
                         "-L", os.path.join(ROOT, "litiv_amd"), "-llitiv_bgs", "-o", str(exe)],
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr[-2000:]
+
+
+def test_python_mirror_of_the_noalloc_pixel_limit():
+    text = open(os.path.join(ROOT, "include", "litiv_bgs.h")).read()
+    m = re.search(r"#define LVBGS_BINCLASSIF_NOALLOC_MAX_PIXELS (\d+)ull", text)
+    assert m and int(m.group(1)) == _bgs().BINCLASSIF_NOALLOC_MAX_PIXELS
+    m = re.search(r"#define LVBGS_ERR_TOO_LARGE \((-\d+)\)", text)
+    assert m and int(m.group(1)) == _bgs().ERR_TOO_LARGE
+
