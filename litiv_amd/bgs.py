@@ -285,7 +285,10 @@ class BackgroundSubtractor:
             self._h = None
 
     def __del__(self):
-        self.close()
+        try:
+            self.close()
+        except Exception:  # interpreter shutdown: the library may already be gone
+            pass
 
     def _check(self, rc):
         if rc < 0:
@@ -489,7 +492,10 @@ class Batch:
             self._h = None
 
     def __del__(self):
-        self.close()
+        try:
+            self.close()
+        except Exception:  # interpreter shutdown: the library may already be gone
+            pass
 
     def _check(self, rc):
         if rc < 0:
@@ -597,7 +603,10 @@ class Server:
             self._h = None
 
     def __del__(self):
-        self.close_server()
+        try:
+            self.close_server()
+        except Exception:  # interpreter shutdown: the library may already be gone
+            pass
 
     def _check(self, rc):
         if rc < 0:
@@ -785,4 +794,7 @@ class PinnedBuffer:
             self._p = None
 
     def __del__(self):
-        self.free()
+        try:
+            self.free()
+        except Exception:  # interpreter shutdown: the library may already be gone
+            pass

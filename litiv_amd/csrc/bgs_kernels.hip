@@ -298,25 +298,12 @@ LV_D bool test_sample(const PxCtx<C>& px, const uint8_t* lut, const uint32_t col
             cdv[c] = absdiff(px.cur[c], col[c]);
             if (cdv[c] > px.t_sc) return false;
         }
-        // lower bounds from the intra distances alone (the inter distance is >= 0): a sample they
-        // already fail needs no inter descriptor
-        uint32_t hd[3], lb_d = 0, lb_s = 0;
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            hd[c] = popc(px.intra[c] ^ desc[c]);
-            uint32_t s = ((hd[c] >> 1) >> 1) * 15u + cdv[c];
-            s = s > 255u ? 255u : s;
-            if (s > px.t_sc) return false;
-            lb_d += hd[c] >> 1;
-            lb_s += s;
-        }
-        if (lb_d > px.t_tot_desc || lb_s > px.t_tot_col) return false;
         uint32_t td = 0, ts = 0;
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             const uint32_t cd = cdv[c];
             const uint32_t inter = lbsp_threshold_packed(px.taps[c], col[c], lut[col[c]]);
-            const uint32_t dd = (hd[c] + popc(inter ^ desc[c])) >> 1;
+            const uint32_t dd = (popc(px.intra[c] ^ desc[c]) + popc(inter ^ desc[c])) >> 1;
             uint32_t sum = (dd >> 1) * 15u + cd;
             sum = sum > 255u ? 255u : sum;
             if (sum > px.t_sc) return false;
@@ -330,13 +317,8 @@ LV_D bool test_sample(const PxCtx<C>& px, const uint8_t* lut, const uint32_t col
     } else if constexpr (ALGO == kSuBSENSE && C == 1) {
         const uint32_t cd = absdiff(px.cur[0], col[0]);
         if (cd > px.t_col) return false;
-        // lower bounds from the intra distance alone (the inter distance is >= 0)
-        const uint32_t hd = popc(px.intra[0] ^ desc[0]);
-        if ((hd >> 1) > px.t_desc) return false;
-        uint32_t lb = ((hd >> 1) >> 2) * 15u + cd;
-        if ((lb > 255u ? 255u : lb) > px.t_col) return false;
         const uint32_t inter = lbsp_threshold_packed(px.taps[0], col[0], lut[col[0]]);
-        const uint32_t dd = (hd + popc(inter ^ desc[0])) >> 1;
+        const uint32_t dd = (popc(px.intra[0] ^ desc[0]) + popc(inter ^ desc[0])) >> 1;
         if (dd > px.t_desc) return false;
         uint32_t sum = (dd >> 2) * 15u + cd;
         sum = sum > 255u ? 255u : sum;
