@@ -21,6 +21,7 @@ fix-point statistics, serving runs. Plain single-stream workloads (1080p, QVGA, 
 thermal) are selectable.
 """
 import argparse
+import collections
 import json
 import os
 import platform
@@ -450,7 +451,9 @@ def run_serve(args, n_slots, budget_gb=250.0, ingest="device", log_prefix="serve
     objects move continuously for the whole lifetime with no jump back. ingest = "device": the frames
     are in HBM and handed over as device pointers; "host": they are in pinned host memory and go
     through the server's ingest path (copied on its copy stream into the stream's device slots as
-    they arrive), so the latency includes the host-to-device copy."""
+    they arrive), so the latency includes the host-to-device copy. Replacement streams are prepared
+    (frames generated, and for host ingest copied to pinned memory) on a background thread, so the
+    serving loop never stalls on their setup."""
     import heapq
 
     import torch
@@ -462,42 +465,56 @@ def run_serve(args, n_slots, budget_gb=250.0, ingest="device", log_prefix="serve
     fps, mean_life = 30.0, 2000.0
     ring = args.serve_host_ring if host else args.serve_ring
     period = 2 * ring - 2
-    rng = np.random.default_rng(0x5E12E)
+    rng = np.random.default_rng(0x5E12E)      # arrivals (serving loop)
+    life_rng = np.random.default_rng(0x11FE)  # lifetimes (stream preparation)
     srv = Server(hbm_budget=int(budget_gb * (1 << 30)), max_inflight=2, max_batch=n_slots, fix_px_per_block=8192,
                  cont_px_per_block=2048, seq_limit=8, groups=2, host_ring=4 if host else 0)
-    backlog = []    # host frames refused while their stream's ingest slots were busy: (arrival, sid, index, tag)
     dev = torch.device("cuda:0")
     incarnation = [0]
     slots = {}      # server stream id -> dict
-    waiting = []    # stream configs that did not fit the budget yet
+    waiting = []    # prepared streams that did not fit the budget yet
+    gen_stream = torch.cuda.Stream()
 
-    def new_stream():
+    def prepare():
+        """A new stream with its frames generated on the GPU (on a stream of its own) and, for host
+        ingest, copied into pinned host memory. Runs on the preparation thread during the window, so
+        the serving loop never waits for it."""
         k = incarnation[0]
         incarnation[0] += 1
         cfg = synth.preset(synth.SERVE, k)
         cfg.frames = max(cfg.frames, ring)
         seq = synth.Sequence(cfg)
-        return {"k": k, "seq": seq, "w": cfg.width, "h": cfg.height,
-                "life": max(1, int(rng.exponential(mean_life)))}
+        st = {"k": k, "seq": seq, "w": cfg.width, "h": cfg.height, "life": max(1, int(life_rng.exponential(mean_life)))}
+        with torch.cuda.stream(gen_stream):
+            frames = torch.empty((ring, seq.frame_bytes), dtype=torch.uint8, device=dev)
+            seq.frames_device(0, ring, frames.data_ptr(), stream=gen_stream.cuda_stream)
+            if host:
+                pinned = PinnedBuffer(ring * seq.frame_bytes)
+                torch.from_numpy(pinned.array).copy_(frames.view(-1), non_blocking=True)
+        gen_stream.synchronize()
+        st["frames"] = pinned if host else frames
+        return st
+
+    ready = collections.deque()   # prepared streams for replacements
+    prep_stop = threading.Event()
+
+    def prep_loop():
+        while not prep_stop.is_set():
+            if len(ready) < 8:
+                ready.append(prepare())
+            else:
+                time.sleep(0.002)
 
     def admit(st, now):
         seq = st["seq"]
-        frames = torch.empty((ring, seq.frame_bytes), dtype=torch.uint8, device=dev)
-        seq.frames_device(0, ring, frames.data_ptr())
-        torch.cuda.synchronize()
+        frames = st["frames"]
         if host:
-            pinned = PinnedBuffer(ring * seq.frame_bytes)
-            pinned.array[:] = frames.view(-1).cpu().numpy()
-            del frames
-            frames = pinned
-            sid = srv.open_host(pinned.array[:seq.frame_bytes].reshape(st["h"], st["w"], 3), stream_id=st["k"])
+            sid = srv.open_host(frames.array[:seq.frame_bytes].reshape(st["h"], st["w"], 3), stream_id=st["k"])
         else:
             sid = srv.open(frames.data_ptr(), st["w"], st["h"], 3, stream_id=st["k"])
         if sid is None:
-            if host:
-                frames.free()
             return None
-        st.update(sid=sid, frames=frames, pend=[], ending=False,
+        st.update(sid=sid, pend=[], ending=False,
                   masks=torch.empty((ring, st["w"] * st["h"]), dtype=torch.uint8, device=dev),
                   t=0, done=0, closing=False, next=now + rng.exponential(1.0 / fps))
         slots[sid] = st
@@ -505,11 +522,14 @@ def run_serve(args, n_slots, budget_gb=250.0, ingest="device", log_prefix="serve
 
     t0 = time.perf_counter()
     for _ in range(n_slots):
-        st = new_stream()
+        st = prepare()
         if admit(st, t0) is None:
             waiting.append(st)
+    torch.cuda.synchronize()
     log("%s: %d streams admitted, %d waiting, %.1f GB of models, setup %.1f s" % (
         log_prefix, len(slots), len(waiting), srv.stats()["bytes_used"] / 1e9, time.perf_counter() - t0))
+    prep = threading.Thread(target=prep_loop, daemon=True)
+    prep.start()
     start = time.perf_counter()
     # the arrival clock starts now
     for sid, st in slots.items():
@@ -521,6 +541,7 @@ def run_serve(args, n_slots, budget_gb=250.0, ingest="device", log_prefix="serve
     lat, px_done, n_done_window = [], 0, 0
     npx = {}
     replaced = 0
+    to_replace = 0   # ended streams whose replacement is not prepared yet
     tag_seq = 0
     last_log = start
     while True:
@@ -578,9 +599,14 @@ def run_serve(args, n_slots, budget_gb=250.0, ingest="device", log_prefix="serve
         # streams that ended and finished: replace them (the new stream may have to wait)
         for sid in [s for s, st in slots.items() if st["closing"]]:
             if srv.retired(sid):
-                del slots[sid]
-                waiting.append(new_stream())
+                st = slots.pop(sid)
+                if host:
+                    st["frames"].free()
+                to_replace += 1
                 replaced += 1
+        while to_replace and ready:
+            waiting.append(ready.popleft())
+            to_replace -= 1
         while waiting:
             sid = admit(waiting[0], time.perf_counter())
             if sid is None:
@@ -595,6 +621,8 @@ def run_serve(args, n_slots, budget_gb=250.0, ingest="device", log_prefix="serve
                                                                    sst["steps"]))
     window = t_end - t_warm
     sst = srv.stats()
+    prep_stop.set()
+    prep.join()
     srv.drain()
     srv.close_server()
     lat_ms = np.array(lat) * 1e3
@@ -603,7 +631,7 @@ def run_serve(args, n_slots, budget_gb=250.0, ingest="device", log_prefix="serve
         "streams": n_slots, "budget_gb": budget_gb, "frames": "%d-frame ping-pong per stream" % ring,
         "ingest": "pinned host memory through the server's copy stream (latency includes the copy)" if host
                   else "device frames",
-        "open_at_end": sst["open_streams"], "waiting_at_end": len(waiting),
+        "open_at_end": sst["open_streams"], "waiting_at_end": len(waiting) + to_replace,
         "replaced": replaced, "window_s": round(window, 1),
         "latency_ms": {"p50": round(float(np.percentile(lat_ms, 50)), 3) if len(lat_ms) else None,
                        "p99": round(float(np.percentile(lat_ms, 99)), 3) if len(lat_ms) else None,

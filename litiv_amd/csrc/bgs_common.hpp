@@ -47,42 +47,10 @@ LV_HD void draw4(uint32_t seed_lo, uint32_t seed_hi, uint32_t stream, uint32_t f
     out[3] >>= 1;
 }
 
-// Remainders of 31-bit draws by small divisors without an integer division (Granlund and
-// Montgomery, PLDI'94, theorem 4.2): with l = ceil(log2 m) and M = ceil(2^(31+l) / m), m*M lies in
-// [2^(31+l), 2^(31+l) + 2^l], so floor(d / m) = floor(d*M / 2^(31+l)) for every d < 2^31; M < 2^32.
-// The quotient is the high word of d*M shifted right by l-1 (m >= 2; m = 1 divides everything).
-struct DivMagic {
-    uint32_t M, sh;
-};
-constexpr int kMagicMax = 1024;  // learning rates (ceil(T) <= 512) and their diffusion spreads
-struct DivMagicTable {
-    DivMagic t[kMagicMax + 1];
-    constexpr DivMagicTable() : t() {
-        for (int m = 1; m <= kMagicMax; ++m) {
-            int l = 0;
-            while ((1 << l) < m) ++l;
-            const unsigned long long M = ((1ull << (31 + l)) + (unsigned long long)m - 1ull) / (unsigned long long)m;
-            t[m].M = (uint32_t)M;
-            t[m].sh = l > 0 ? (uint32_t)(l - 1) : 0u;
-        }
-    }
-};
-LV_HD DivMagic div_magic(uint32_t m) {
-    // one constant table per program, read with the divisor as index
-    static constexpr DivMagicTable kTab{};
-    return kTab.t[m];
-}
-// d % m for d < 2^31, 1 <= m, with the divisor's magic numbers
-LV_HD uint32_t mod_magic(uint32_t d, uint32_t m, DivMagic g) {
-    const uint32_t q = m == 1u ? d : (uint32_t)(((uint64_t)d * g.M) >> 32) >> g.sh;
-    return d - q * m;
-}
-
 // `draw % m` for the reference's size_t moduli: m can be SIZE_MAX (infinite learning rate),
 // and a 31-bit draw is then smaller than m, so the result is the draw itself.
 LV_HD uint32_t mod64(uint32_t d, uint64_t m) {
     if (m > 0xFFFFFFFFull) return d;
-    if (m <= (uint64_t)kMagicMax && m > 0) return mod_magic(d, (uint32_t)m, div_magic((uint32_t)m));
     return d % (uint32_t)m;
 }
 
