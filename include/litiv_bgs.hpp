@@ -101,6 +101,17 @@ public:
         return out;
     }
 
+    /* Checkpoint / resume of the whole model as binary matrix archives (lvbgs_checkpoint_save / _load);
+     * no counterpart in the reference, which keeps its model in memory only. save needs an existing
+     * directory; load needs a subtractor constructed with the same parameters and stream id. */
+    void saveCheckpoint(const std::string& dir) const { check(lvbgs_checkpoint_save(ctx_, dir.c_str())); }
+    void loadCheckpoint(const std::string& dir) {
+        check(lvbgs_checkpoint_load(ctx_, dir.c_str()));
+        lvbgs_stats s;
+        check(lvbgs_get_stats(ctx_, &s));
+        w_ = (int)s.width, h_ = (int)s.height, c_ = (int)s.channels;
+    }
+
     lvbgs_ctx* handle() const { return ctx_; }
     void* stream() const { return lvbgs_stream(ctx_); }
 
