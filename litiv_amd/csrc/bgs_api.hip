@@ -29,7 +29,7 @@ struct Arena {
     template <typename T>
     T* take(size_t count) {
         off = (off + 255) & ~(size_t)255;
-        T* p = reinterpret_cast<T*>(base + off);
+        T* p = base ? reinterpret_cast<T*>(base + off) : nullptr;  // sizing pass: offsets only
         off += count * sizeof(T);
         return p;
     }
