@@ -17,9 +17,11 @@
 //
 // Host ingest (options.host_ring > 0): every stream's arena holds host_ring frame slots. A frame
 // submitted from host memory is copied at once into the stream's next free slot on the server's
-// copy stream, with an event after it, so the copy overlaps the steps already running; the step
-// that takes the frame waits for that event on the device. Slots are freed in order as the
-// stream's frames complete; a stream with every slot busy refuses more (LVBGS_ERR_BUSY).
+// copy stream, with an event after it, so the copy overlaps the steps already running. A step
+// takes a host frame only once its copy has completed (its stream sits that step out otherwise),
+// so the copies queued last never hold up the frames already in HBM; the step still waits for the
+// event on the device. Slots are freed in order as the stream's frames complete; a stream with
+// every slot busy refuses more (LVBGS_ERR_BUSY).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -292,6 +294,14 @@ static int launch_step(lvbgs_server* s) {
             if (e != hipSuccess) return srv_err(s, LVBGS_ERR_HIP, "stream initialization failed");
             sl.ready = true;
         }
+        // a frame still being copied from host memory waits for a later step, so that the copies
+        // queued last do not hold up every frame of this one
+        const int rs = sl.pending.front().ring_slot;
+        if (rs >= 0) {
+            const hipError_t e = hipEventQuery(s->ring_ev[(size_t)i][(size_t)rs]);
+            if (e == hipErrorNotReady) continue;
+            if (e != hipSuccess) return srv_err(s, LVBGS_ERR_HIP, "ingest copy failed");
+        }
         members.push_back(i);
     }
     if (members.empty()) return LVBGS_OK;
@@ -431,6 +441,9 @@ extern "C" int lvbgs_server_drain(lvbgs_server* s) {
             bool waiting = false;
             for (auto& sl : s->slots) waiting = waiting || (sl.open && !sl.pending.empty());
             if (!waiting) break;
+            // nothing could launch: the waiting frames' copies (or initializations) are still running
+            for (auto cs : s->cstream)
+                if (cs && hipStreamSynchronize(cs) != hipSuccess) return srv_err(s, LVBGS_ERR_HIP, "ingest copy");
             continue;
         }
         if ((rc = harvest(s, true))) return rc;
