@@ -334,7 +334,9 @@ long long lvbgs_archive_read(const char* path, int* cv_type, int* dims, int32_t*
  * restart. */
 int lvbgs_checkpoint_save(lvbgs_ctx* ctx, const char* dir);
 /* Restores a checkpoint into a context created with the same algorithm, parameters, rng_seed and
- * stream_id (else LVBGS_ERR_ARG): the context then continues the saved sequence bit for bit. */
+ * stream_id (else LVBGS_ERR_ARG): the context then continues the saved sequence bit for bit. A load
+ * that fails after reading the geometry (a missing or truncated buffer) leaves the context
+ * uninitialized: initialize it or load a checkpoint again. */
 int lvbgs_checkpoint_load(lvbgs_ctx* ctx, const char* dir);
 
 #ifdef __cplusplus

@@ -729,20 +729,6 @@ extern "C" long long lvbgs_get_state(lvbgs_ctx* c, const char* name, void* out, 
 }
 
 // ---- checkpoint restore (bgs_archive.hip): geometry first, then the saved buffers, then the scalars ----
-// samples are staged on the host in the device record layout until the scalars arrive
-static std::vector<uint32_t>& restore_records(lvbgs_ctx* c) {
-    static thread_local std::vector<std::pair<lvbgs_ctx*, std::vector<uint32_t>>> staged;
-    for (auto& e : staged)
-        if (e.first == c) return e.second;
-    staged.emplace_back(c, std::vector<uint32_t>());
-    return staged.back().second;
-}
-
-static void drop_restore_records(lvbgs_ctx* c) {
-    std::vector<uint32_t>& r = restore_records(c);
-    std::vector<uint32_t>().swap(r);
-}
-
 int lvbgs_restore_begin(lvbgs_ctx* c, int w, int h, int ch, const uint8_t* roi, size_t n_roi, size_t orig_roi) {
     (void)hipSetDevice(c->device);
     const size_t npx = (size_t)w * (size_t)h;
@@ -760,7 +746,7 @@ int lvbgs_restore_begin(lvbgs_ctx* c, int w, int h, int ch, const uint8_t* roi, 
                         "roi")))
         return rc;
     const size_t rec = c->C == 3 ? 3 : 1, N = c->p.bg_samples;
-    std::vector<uint32_t>& r = restore_records(c);
+    std::vector<uint32_t>& r = c->restore_stage;
     r.assign((N * ((npx + 63) / 64) + kMaxFetch) * 64 * rec, 0u);
     return hip_check(c, hipStreamSynchronize(c->stream), "restore");
 }
@@ -772,7 +758,7 @@ int lvbgs_restore_buffer(lvbgs_ctx* c, const char* name, const void* data, size_
     if (n == "samples_color" || n == "samples_desc") {
         // the oracle layout (N x H x W x C) into the tile-major records (sample s of 64-pixel group g
         // at record (g*N + s)*64 + lane); RGB {d0 | d1<<16, d2 | c0<<16 | c1<<24, c2}, grey {d0 | c0<<16}
-        std::vector<uint32_t>& r = restore_records(c);
+        std::vector<uint32_t>& r = c->restore_stage;
         const bool col = n == "samples_color";
         if (bytes != N * npx * C * (col ? 1 : 2) || r.empty()) return set_err(c, LVBGS_ERR_ARG, "sample planes");
         const size_t rw = C == 3 ? 3 : 1;
@@ -822,9 +808,10 @@ int lvbgs_restore_buffer(lvbgs_ctx* c, const char* name, const void* data, size_
 }
 
 int lvbgs_restore_scalars(lvbgs_ctx* c, const Scalars& s, uint32_t apply_count) {
-    std::vector<uint32_t>& r = restore_records(c);
+    std::vector<uint32_t>& r = c->restore_stage;
+    if (r.empty()) return set_err(c, LVBGS_ERR_STATE, "checkpoint restore not begun");
     int rc = hip_check(c, hipMemcpy(c->v.sA, r.data(), r.size() * 4, hipMemcpyHostToDevice), "samples");
-    drop_restore_records(c);
+    std::vector<uint32_t>().swap(r);
     if (rc) return rc;
     if ((rc = hip_check(c, hipMemcpy(c->v.sc, &s, sizeof(Scalars), hipMemcpyHostToDevice), "scalars"))) return rc;
     c->apply_count = apply_count;

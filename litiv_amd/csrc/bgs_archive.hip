@@ -217,6 +217,8 @@ int lvbgs_restore_begin(lvbgs_ctx* c, int w, int h, int ch, const uint8_t* roi, 
 int lvbgs_restore_buffer(lvbgs_ctx* c, const char* name, const void* data, size_t bytes);
 int lvbgs_restore_scalars(lvbgs_ctx* c, const Scalars& s, uint32_t apply_count);
 
+static int load_buffers(lvbgs_ctx* c, const char* dir, int W, int H, int C, const std::vector<double>& meta);
+
 extern "C" int lvbgs_checkpoint_load(lvbgs_ctx* c, const char* dir) {
     if (!c || !dir) return LVBGS_ERR_ARG;
     c->err.clear();
@@ -238,13 +240,23 @@ extern "C" int lvbgs_checkpoint_load(lvbgs_ctx* c, const char* dir) {
         return set_err(c, LVBGS_ERR_ARG, "checkpoint of a model with other parameters or RNG key");
     const int W = (int)meta[kMetaW], H = (int)meta[kMetaH], C = (int)meta[kMetaC];
     if (W <= 0 || H <= 0 || (C != 1 && C != 3)) return set_err(c, LVBGS_ERR_ARG, "bad checkpoint geometry");
-    const size_t npx = (size_t)W * (size_t)H, N = p.bg_samples;
-    const bool sub = c->algo == LVBGS_SUBSENSE;
+    const size_t npx = (size_t)W * (size_t)H;
     std::vector<uint8_t> roi(npx);
     if (read_archive(join(dir, "roi"), d, roi.data(), roi.size()) != (long long)npx)
         return set_err(c, LVBGS_ERR_ARG, std::string("cannot read ") + join(dir, "roi"));
     int rc = lvbgs_restore_begin(c, W, H, C, roi.data(), (size_t)meta[kMetaNRoi], (size_t)meta[kMetaOrigRoi]);
     if (rc) return rc;
+    rc = load_buffers(c, dir, W, H, C, meta);
+    if (rc) std::vector<uint32_t>().swap(c->restore_stage);  // the context stays uninitialized
+    return rc;
+}
+
+// the saved buffers and scalars of a checkpoint whose geometry lvbgs_restore_begin has set up
+static int load_buffers(lvbgs_ctx* c, const char* dir, int W, int H, int C, const std::vector<double>& meta) {
+    const size_t npx = (size_t)W * (size_t)H, N = c->p.bg_samples;
+    const bool sub = c->algo == LVBGS_SUBSENSE;
+    ArrayDesc d;
+    int rc;
     std::vector<uint8_t> buf;
     for (const Item& it : kItems) {
         if (it.subsense_only && !sub) continue;

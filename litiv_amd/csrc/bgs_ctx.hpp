@@ -31,6 +31,9 @@ struct lvbgs_ctx {
     uint8_t* d_ring = nullptr;   // ring_frames frames of W*H*C bytes (in the arena)
     lvbgs::StepOptions opt{};
     uint32_t apply_count = 0;
+    // checkpoint load: the sample planes repacked into the device record layout on the host, held
+    // between the geometry set-up and the upload of the scalars
+    std::vector<uint32_t> restore_stage;
     // timing
     int timed_stage = 0;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pairs;

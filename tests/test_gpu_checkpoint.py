@@ -2,14 +2,15 @@
 archive per state buffer) and loaded into a fresh context (lvbgs_checkpoint_load) continues the
 sequence exactly. Every mask of the resumed run equals the uninterrupted run's and the CPU oracle's
 (which never stops), and the final model states are identical. Contexts whose arena was sized for
-another frame, ROIs and one-channel frames are covered, and mismatched parameters are refused."""
+another frame, ROIs and one-channel frames are covered, mismatched parameters are refused, and a load
+that fails part-way leaves the context uninitialized."""
 import os
 
 import numpy as np
 import pytest
 
 from litiv_amd import BackgroundSubtractorLOBSTER, BackgroundSubtractorSuBSENSE, synth
-from litiv_amd.bgs import LvbgsError, read_archive
+from litiv_amd.bgs import ERR_STATE, LvbgsError, read_archive
 from oracle_lib import LOBSTER, SUBSENSE, Oracle
 
 pytestmark = pytest.mark.gpu
@@ -99,6 +100,18 @@ def test_checkpoint_of_a_model_with_other_parameters_is_refused(tmp_path):
             b.load_checkpoint(ck)
     with pytest.raises(LvbgsError):
         BackgroundSubtractorLOBSTER(stream_id=1).load_checkpoint(ck)
+    # a checkpoint missing one buffer: the load fails after the geometry, the context is left
+    # uninitialized, and a complete checkpoint loads into it afterwards
+    import shutil
+    broken = str(tmp_path / "broken")
+    shutil.copytree(ck, broken)
+    os.remove(os.path.join(broken, "T.bin"))
     b = BackgroundSubtractorSuBSENSE(stream_id=1)
+    b.initialize(f0)
+    with pytest.raises(LvbgsError):
+        b.load_checkpoint(broken)
+    with pytest.raises(LvbgsError) as e:
+        b.apply(seq.frame(2))   # reaches the library, which reports the uninitialized model
+    assert e.value.code == ERR_STATE
     b.load_checkpoint(ck)
     assert np.array_equal(b.apply(seq.frame(2)), a.apply(seq.frame(2)))
