@@ -411,6 +411,12 @@ def run_batch(args, preset, n_streams, ref_frames):
                      "mean_worklist_per_stream": [round(float(x), 1) for x in (work / len(iters))[1:6]]},
     }
     if args.serve_seconds > 0:
+        # the batch's frames (tens of GB) and models leave HBM before the serving runs, whose budgets
+        # assume the device to themselves
+        for b in subs:
+            b.close()
+        del frames, masks, subs
+        torch.cuda.empty_cache()
         result["serving"] = {}
         runs = [(int(x), 250.0, "device") for x in str(args.serve_streams).split(",") if x]
         runs += [(int(x.split(":")[0]), float(x.split(":")[1]), "device") for x in str(args.serve_bound).split(",")
@@ -421,6 +427,7 @@ def run_batch(args, preset, n_streams, ref_frames):
                 n, gb, ingest, args.serve_warmup, args.serve_seconds))
             key = "%d_streams" % n + ("" if gb == 250.0 else "_%dGB" % gb) + ("_host_ingest" if ingest == "host" else "")
             result["serving"][key] = run_serve(args, n, gb, ingest)
+            torch.cuda.empty_cache()
         log("serving: %s" % json.dumps(result["serving"]))
     if not args.no_cpu:
         # streams 16g .. 16g+threads-1 of each resolution class g, one per thread, first frames
@@ -775,7 +782,7 @@ def main():
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="subsense_batch64", choices=sorted(WORKLOADS))
-    ap.add_argument("--cpu-budget", type=float, default=16.0, help="seconds of oracle time for the CPU baseline")
+    ap.add_argument("--cpu-budget", type=float, default=24.0, help="seconds of oracle time for the CPU baseline")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1),
                     help="batched workload: host threads of the CPU baseline (one stream each)")
     ap.add_argument("--meas-steps", type=int, default=60, help="steps of each measurement pass")
@@ -786,9 +793,9 @@ def main():
                     help="batched workload: steps of the tuning pass run before the timed region")
     ap.add_argument("--tune-period", type=int, default=8, help="tuning pass: steps between trial rounds")
     ap.add_argument("--calib-copy-mb", type=int, default=0, help="profiling runs: one device copy of this size first")
-    ap.add_argument("--serve-seconds", type=float, default=20.0,
+    ap.add_argument("--serve-seconds", type=float, default=12.0,
                     help="batched workload: measured seconds of the serving run (0 = skip)")
-    ap.add_argument("--serve-warmup", type=float, default=5.0)
+    ap.add_argument("--serve-warmup", type=float, default=3.0)
     ap.add_argument("--serve-streams", default="256,200", help="comma-separated stream-slot counts, one run each")
     ap.add_argument("--serve-bound", default="256:100",
                     help="comma-separated slots:GB serving runs with a smaller model budget (admission binds)")

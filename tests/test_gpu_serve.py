@@ -290,34 +290,45 @@ def test_device_error_is_reported_for_one_stream_only():
 
 
 def test_host_ingest_matches_device_frames():
-    """Frames submitted from pinned host memory (lvbgs_server_submit_host: copied at once into the
-    stream's ingest slots on the copy stream, waited for on the device by the step that takes them)
-    give the oracle's masks, mixed in the same steps with a stream fed device frames; a stream whose
-    slots are all busy refuses more until frames complete."""
+    """Frames submitted from host memory (lvbgs_server_submit_host) give the oracle's masks, mixed in
+    the same steps with a stream fed device frames: from pinned memory and from pageable memory, both
+    with padded rows (copied into the stream's ingest slot on the server's copy stream; a step takes
+    a frame once its copy has completed). A stream whose slots are all busy refuses more until frames
+    complete."""
     from litiv_amd.bgs import PinnedBuffer
     torch = _torch()
     n = 12
-    hs = Stream(torch, 5, 128, 96, 3, n)       # host-fed
-    ds = Stream(torch, 6, 96, 72, 3, n)        # device-fed
+    hs = Stream(torch, 5, 128, 96, 3, n)       # pinned host memory
+    ps = Stream(torch, 7, 80, 64, 3, n)        # pageable host memory
+    ds = Stream(torch, 6, 96, 72, 3, n)        # device frames
     torch.cuda.synchronize()
     ring = 3
     srv = Server(hbm_budget=1 << 30, max_inflight=2, host_ring=ring)
     assert srv.stream_bytes(128, 96, 3) >= Server.model_bytes(128, 96, 3) + ring * 128 * 96 * 3
-    pinned = PinnedBuffer(n * hs.frames[0].nbytes)
-    fb = hs.frames[0].nbytes
+    row, stride = 128 * 3, 128 * 3 + 8        # padded rows
+    fb = stride * 96
+    pinned = PinnedBuffer(n * fb)
     for t in range(n):
-        pinned.array[t * fb:(t + 1) * fb] = hs.frames[t].reshape(-1)
+        pinned.array[t * fb:(t + 1) * fb].reshape(96, stride)[:, :row] = hs.frames[t].reshape(96, row)
+    prow, pstride = 80 * 3, 80 * 3 + 12
+    pageable = [np.zeros((64, pstride), np.uint8) for _ in range(n)]
+    for t in range(n):
+        pageable[t][:, :prow] = ps.frames[t].reshape(64, prow)
     sh = srv.open_host(hs.frames[0], **hs.params)
+    sp = srv.open_host(ps.frames[0], **ps.params)
     sd = _open(srv, ds)
-    statuses, t_h, t_d = {}, 0, 0
+    statuses, t_h, t_p, t_d = {}, 0, 0, 0
     # no step runs before a pump: the ring fills and the next host frame is refused
     for t_h in range(ring):
-        assert srv.submit_host(sh, pinned.ptr + t_h * fb, hs.m[t_h].data_ptr(), 0.0, tag=t_h)
+        assert srv.submit_host(sh, pinned.ptr + t_h * fb, hs.m[t_h].data_ptr(), 0.0, tag=t_h, stride=stride)
     t_h = ring
-    assert not srv.submit_host(sh, pinned.ptr + t_h * fb, hs.m[t_h].data_ptr(), 0.0, tag=t_h)
-    while t_h < n or t_d < n:
-        if t_h < n and srv.submit_host(sh, pinned.ptr + t_h * fb, hs.m[t_h].data_ptr(), 0.0, tag=t_h):
+    assert not srv.submit_host(sh, pinned.ptr + t_h * fb, hs.m[t_h].data_ptr(), 0.0, tag=t_h, stride=stride)
+    while t_h < n or t_p < n or t_d < n:
+        if t_h < n and srv.submit_host(sh, pinned.ptr + t_h * fb, hs.m[t_h].data_ptr(), 0.0, tag=t_h, stride=stride):
             t_h += 1
+        if t_p < n and srv.submit_host(sp, pageable[t_p].ctypes.data, ps.m[t_p].data_ptr(), 0.0, tag=200 + t_p,
+                                       stride=pstride):
+            t_p += 1
         if t_d < n:
             srv.submit(sd, ds.d[t_d].data_ptr(), ds.m[t_d].data_ptr(), 0.0, tag=100 + t_d)
             t_d += 1
@@ -327,10 +338,11 @@ def test_host_ingest_matches_device_frames():
     srv.drain()
     for tag, sid, st in srv.poll():
         statuses[int(tag)] = int(st)
-    assert len(statuses) == 2 * n and all(v == 0 for v in statuses.values())
+    assert len(statuses) == 3 * n and all(v == 0 for v in statuses.values())
     torch.cuda.synchronize()
     for t in range(n):
-        assert np.array_equal(hs.m[t].cpu().numpy(), hs.expected[t]), "host-fed frame %d" % t
-        assert np.array_equal(ds.m[t].cpu().numpy(), ds.expected[t]), "device-fed frame %d" % t
+        assert np.array_equal(hs.m[t].cpu().numpy(), hs.expected[t]), "pinned host frame %d" % t
+        assert np.array_equal(ps.m[t].cpu().numpy(), ps.expected[t]), "pageable host frame %d" % t
+        assert np.array_equal(ds.m[t].cpu().numpy(), ds.expected[t]), "device frame %d" % t
     srv.close_server()
     pinned.free()
