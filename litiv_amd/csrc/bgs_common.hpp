@@ -60,18 +60,28 @@ LV_HD uint32_t mod64(uint32_t d, uint64_t m) {
 constexpr int kLbspDX[16] = {-2, 2, 0, 0, -2, 2, 2, -2, 0, -1, 0, 1, -1, 1, 1, -1};
 constexpr int kLbspDY[16] = {0, 0, -2, 2, 2, -2, 2, -2, 1, 0, -1, 0, -1, 1, -1, 1};
 
-// bit n = (|v[n]-ref| > t), LBSP.hpp:193-224. Packed taps: word w holds taps 4w..4w+3 (byte k = tap 4w+k).
-// v_sad_u8(v, ref, ~t) = |v-ref| + 2^32-1-t wraps past 2^32 (bit 31 clears) exactly when |v-ref| > t
-// (both at most 255); v_alignbit shifts that bit 31 into the result, tap 15 first, so bit n ends
-// up holding tap n's "|v-ref| <= t"; the result is its complement. Two VALU per tap.
+// bit n = (|v[n]-ref| > t), LBSP.hpp:193-224. Packed taps: word w holds taps 4w..4w+3 (byte k = tap
+// 4w+k), four per VGPR. One v_mqsad_u32_u8 per word: window k of the quad SAD starts at byte k of
+// S0, so it sees tap 4w+k in its byte 0; S1 holds the reference in byte 0 and zeros elsewhere, and
+// the masked SAD skips the bytes where S1 is zero, so result k = |tap - ref| + ~t, which wraps past
+// 2^32 (bit 31 clears) exactly when |tap - ref| > t (both at most 255). v_alignbit shifts those bits
+// in, tap 15 first, so bit n ends up holding tap n's "|v-ref| <= t"; the result is its complement.
+// The mask would also skip byte 0 when ref == 0: that case compares the complemented taps with 255
+// instead (255 - v against 255 - 0 has the same absolute difference).
 LV_D uint32_t lbsp_threshold_packed(const uint32_t taps[4], uint32_t ref, uint32_t t) {
+    typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
     const uint32_t acc = ~t;
+    const u32x4 a = {acc, acc, acc, acc};
+    const uint32_t flip = ref == 0u ? 0xFFFFFFFFu : 0u;
+    const uint32_t r = ref ^ (flip & 0xFFu);
     uint32_t d = 0;
 #pragma unroll
-    for (int n = 15; n >= 0; --n) {
-        const uint32_t v = (taps[n >> 2] >> (8 * (n & 3))) & 0xFFu;
-        const uint32_t x = __builtin_amdgcn_sad_u8(v, ref, acc);
-        d = __builtin_amdgcn_alignbit(d, x, 31u);  // (d << 1) | (x >> 31)
+    for (int w = 3; w >= 0; --w) {
+        const u32x4 q = __builtin_amdgcn_mqsad_u32_u8((uint64_t)(taps[w] ^ flip), r, a);
+        d = __builtin_amdgcn_alignbit(d, q[3], 31u);
+        d = __builtin_amdgcn_alignbit(d, q[2], 31u);
+        d = __builtin_amdgcn_alignbit(d, q[1], 31u);
+        d = __builtin_amdgcn_alignbit(d, q[0], 31u);
     }
     return ~d & 0xFFFFu;
 }

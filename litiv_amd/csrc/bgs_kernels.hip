@@ -88,16 +88,27 @@ LV_D size_t sample_index(const View& v, uint32_t s, uint32_t p) {
     return ((size_t)(p >> 6) * (uint32_t)v.N + s) * 64u + (p & 63u);
 }
 
+// A device pointer seen through the global address space. The pointers of a batch member's View are
+// read from a table in device memory, so the compiler sees them as generic (flat) pointers; loads
+// through address space 1 are global loads instead (an SGPR base with a 32-bit lane offset, and
+// LDS waits that do not also wait for them).
+template <typename T>
+using gptr = __attribute__((address_space(1))) T*;
+template <typename T>
+LV_D gptr<T> G(T* p) {
+    return (gptr<T>)p;
+}
+
 // raw sample in registers: ra = record words 0-1 (RGB) or word 0 (grey), rb = word 2 (RGB)
 template <int C>
 LV_D void fetch_raw(const View& v, uint32_t s, uint32_t p, uint64_t& ra, uint32_t& rb) {
     const size_t idx = sample_index(v, s, p);
     if constexpr (C == 3) {
-        const Rec12 r = ((const Rec12*)v.sA)[idx];
-        ra = (uint64_t)r.w0 | ((uint64_t)r.w1 << 32);
-        rb = r.w2;
+        const gptr<const uint32_t> q = G((const uint32_t*)v.sA) + idx * 3u;
+        ra = (uint64_t)q[0] | ((uint64_t)q[1] << 32);
+        rb = q[2];
     } else {
-        ra = ((const uint32_t*)v.sA)[idx];
+        ra = G((const uint32_t*)v.sA)[idx];
         rb = 0;
     }
 }
@@ -109,12 +120,12 @@ LV_D const uint32_t* rec_base(const View& v, uint32_t p) {
 
 template <int C>
 LV_D void fetch_at(const uint32_t* rec, uint64_t& ra, uint32_t& rb) {
+    const gptr<const uint32_t> q = G(rec);
     if constexpr (C == 3) {
-        const Rec12 r = *(const Rec12*)rec;
-        ra = (uint64_t)r.w0 | ((uint64_t)r.w1 << 32);
-        rb = r.w2;
+        ra = (uint64_t)q[0] | ((uint64_t)q[1] << 32);
+        rb = q[2];
     } else {
-        ra = *rec;
+        ra = q[0];
         rb = 0;
     }
 }
@@ -461,7 +472,8 @@ LV_D bool build_repl(const View& v, const uint8_t* frame, int x, int y, uint32_t
         const uint32_t q = (uint32_t)((y + kPredDY[i]) * v.W + (x + kPredDX[i]));
         const uint32_t sl = rec_slot_diff(rec_of(i));
         bool seen = false;
-        for (int k = 0; k < n; ++k) seen = seen || R.slot[k][t] == sl;
+#pragma unroll
+        for (int k = 0; k < kRepl; ++k) seen = seen || (k < n && R.slot[k][t] == sl);
         if (seen) continue;
         if (n == kRepl) return false;
         uint64_t ra;
@@ -942,19 +954,58 @@ LV_D uint32_t applied_writers_tile(const PredTile& t, int lx, int ly) {
     return m;
 }
 
-template <int C, int ALGO, int KF>
+// LDS of the phase-A match for frames of up to CMAX channels. A batch kernel instantiates the match
+// for both channel counts and calls both with CMAX = 3, so they share one allocation (separate
+// arrays per instantiation would cost ~5 KB of LDS and an occupancy level).
+template <int CMAX>
+struct MatchLds {
+    union {
+        FrameTile<CMAX> big;
+        FrameTile<1> c1;
+    } tile;
+    PredTile pred;
+    uint8_t lut[256];
+    uint16_t perm[TPB];
+    uint32_t nfg[TPB / 64];
+    union {
+        PxIn<CMAX> big[TPB];
+        PxIn<1> c1[TPB];
+    } in;
+};
+
+template <int CMAX>
+LV_D MatchLds<CMAX>& match_lds() {
+    __shared__ MatchLds<CMAX> m;
+    return m;
+}
+
+template <int C, int CMAX>
+LV_D FrameTile<C>& tile_of(MatchLds<CMAX>& m) {
+    if constexpr (C == CMAX) return m.tile.big;
+    else return m.tile.c1;
+}
+
+template <int C, int CMAX>
+LV_D PxIn<C>* px_in_of(MatchLds<CMAX>& m) {
+    if constexpr (C == CMAX) return m.in.big;
+    else return m.in.c1;
+}
+
+template <int C, int ALGO, int KF, int CMAX = C>
 LV_D void match_body(const View& v, uint32_t bid, uint32_t nblk) {
+    static_assert(C == CMAX || C == 1, "a match instantiation shares the LDS of CMAX channels");
     const uint8_t* frame = v.frame;
     const uint32_t seq_limit = v.seq_limit;
-    __shared__ FrameTile<C> tile;
-    __shared__ PredTile pred;
-    __shared__ uint8_t lut[256];
+    MatchLds<CMAX>& L = match_lds<CMAX>();
+    FrameTile<C>& tile = tile_of<C>(L);
+    PredTile& pred = L.pred;
+    uint8_t* lut = L.lut;
     const int ntx = (v.W + TW - 1) / TW;
     const int tid = (int)bid;
     const int x0 = (tid % ntx) * TW, y0 = (tid / ntx) * TH;
-    __shared__ uint16_t s_perm[TPB];
-    __shared__ uint32_t s_nfg[TPB / 64];
-    __shared__ PxIn<C> s_in[TPB];
+    uint16_t* s_perm = L.perm;
+    uint32_t* s_nfg = L.nfg;
+    PxIn<C>* s_in = px_in_of<C>(L);
     // this thread's tile pixel before the regrouping: its inputs are loaded alongside the tile
     const int ox = x0 + (int)threadIdx.x % TW, oy = y0 + (int)threadIdx.x / TW;
     const bool oin = ox < v.W && oy < v.H;
@@ -2365,12 +2416,13 @@ LV_D void for_batch_block(const BatchArgs& b, F&& f) {
 
 template <int C, int ALGO>
 __global__ void __launch_bounds__(TPB) k_describe(View v) { describe_body<C, ALGO>(v, LV_SINGLE_BID); }
-// occupancy 5 (at most 96 VGPRs) fits both fetch depths without spilling
-#define LV_MATCH_WAVES(KF) __attribute__((amdgpu_waves_per_eu(5)))
+// occupancy 6 (at most 80 VGPRs, 26 KB of LDS per 256-thread block) fits both fetch depths without spilling
+#define LV_MATCH_WAVES(KF) __attribute__((amdgpu_waves_per_eu(6)))
 template <int C, int ALGO, int KF>
 __global__ void __launch_bounds__(TPB) LV_MATCH_WAVES(KF) k_match(View v) { match_body<C, ALGO, KF>(v, LV_SINGLE_BID); }
+// phase B holds 80 VGPRs at occupancy 6 as well
 template <int C, int ALGO, int KF>
-__global__ void __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(4))) k_match_seq(View v) {
+__global__ void __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(6))) k_match_seq(View v) {
     match_seq_body<C, ALGO, KF>(v, LV_SINGLE_BID);
 }
 template <int C, int ALGO>
@@ -2412,9 +2464,9 @@ __global__ void __launch_bounds__(TPB) k_refresh(View v, int use_pending, float 
     })
 __global__ void __launch_bounds__(TPB) kb_describe(BatchArgs b) { LV_BATCH_C(describe_body, kSuBSENSE); }
 template <int KF>
-__global__ void __launch_bounds__(TPB) LV_MATCH_WAVES(KF) kb_match(BatchArgs b) { LV_BATCH_C(match_body, kSuBSENSE, KF); }
+__global__ void __launch_bounds__(TPB) LV_MATCH_WAVES(KF) kb_match(BatchArgs b) { LV_BATCH_C(match_body, kSuBSENSE, KF, 3); }
 template <int KF>
-__global__ void __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(4))) kb_match_seq(BatchArgs b) {
+__global__ void __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(6))) kb_match_seq(BatchArgs b) {
     LV_BATCH_C(match_seq_body, kSuBSENSE, KF);
 }
 __global__ void __launch_bounds__(TPB) kb_match_cont(BatchArgs b) { LV_BATCH_C(match_cont_body, kSuBSENSE); }

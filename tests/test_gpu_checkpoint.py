@@ -36,6 +36,18 @@ def _roi(h, w):
     return r
 
 
+def _parse_archive(path):
+    """Header and element bytes of a MatArchive_BINARY file: int32 type, uint64 element size, uint64
+    element count, int32 dims, int32 sizes, data (modules/utils/src/opencv.cpp:514-531)."""
+    import struct
+    raw = open(path, "rb").read()
+    typ, esz, cnt, nd = struct.unpack_from("<iQQi", raw, 0)
+    sizes = struct.unpack_from("<%di" % nd, raw, 24)
+    body = raw[24 + 4 * nd:]
+    assert len(body) == esz * cnt and cnt == int(np.prod(sizes))
+    return typ, sizes, body
+
+
 def _cls(algo):
     return BackgroundSubtractorSuBSENSE if algo == SUBSENSE else BackgroundSubtractorLOBSTER
 
@@ -67,6 +79,16 @@ def test_checkpoint_resume_is_exact(name, tmp_path):
     a.save_checkpoint(ck)
     meta, _ = read_archive(os.path.join(ck, "meta.bin"))
     assert int(meta[2]) == w and int(meta[3]) == h
+    # every file is a plain binary matrix archive: an independent parse of the bytes gives the state
+    # getters' arrays (the sample planes as N x H x W matrices of C-channel elements)
+    names = SUB_STATE if algo == SUBSENSE else LOB_STATE
+    for nme in names:
+        typ, shape, body = _parse_archive(os.path.join(ck, nme + ".bin"))
+        ref = a.state(nme)
+        depth = {np.dtype(np.uint8): 0, np.dtype(np.uint16): 2, np.dtype(np.float32): 5}[ref.dtype]
+        assert typ & 7 == depth, nme
+        assert body == ref.tobytes(), nme
+        assert int(np.prod(shape)) * ((typ >> 3) + 1) == ref.size, nme
 
     b = _cls(algo)(stream_id=3)
     other = frames[0][: h // 2, : w // 2]

@@ -48,6 +48,20 @@ def test_lbsp_random_vs_oracle(shape, t):
     assert np.array_equal(out, ref)
 
 
+@pytest.mark.parametrize("t", [0, 1, 2, 127, 253, 254, 255, 256])
+def test_lbsp_extreme_values_vs_oracle(t):
+    # centres and taps at the ends of the byte range: a zero centre (which the masked quad SAD
+    # treats through the complemented taps), differences of exactly t and t + 1, and 255 - 0
+    rng = np.random.default_rng(1000 + t)
+    vals = np.array([0, 1, 2, 126, 127, 128, 253, 254, 255], np.uint8)
+    for shape in ((48, 70, 3), (33, 29, 1)):
+        img = vals[rng.integers(0, len(vals), size=shape)]
+        out = _lbsp_gpu(img, t)
+        ref = np.zeros(shape, np.uint16)
+        orc().orc_lbsp_compute_abs(img, shape[1], shape[0], shape[2], t, ref)
+        assert np.array_equal(out, ref), shape
+
+
 @pytest.mark.parametrize("kind,over", [(synth.QVGA, {}), (synth.THERMAL, {}), (synth.BATCH64, {"stream": 50})])
 def test_synth_device_equals_host(kind, over):
     torch = _torch()
