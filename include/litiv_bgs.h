@@ -158,6 +158,12 @@ int lvbgs_set_fixpoint_schedule(lvbgs_ctx* ctx, int launches, int blocks);
  * continuation phases, and the grid of those phases. Results do not depend on it. */
 int lvbgs_set_match_schedule(lvbgs_ctx* ctx, int seq_limit, int cont_blocks);
 
+/* LBSP self-check: lvbgs_lbsp_compute_abs through the form of the threshold test the sample match
+ * uses (taps packed four per register, masked quad SADs) instead of the per-tap form the descriptor
+ * kernels use; the two must give the same bits. */
+int lvbgs_debug_lbsp_packed(const void* d_img, int width, int height, int channels, int threshold, void* d_out,
+                            void* stream);
+
 /* RNG self-check: the four draws of group 0 of counter (pixel p, frame, phase, event) for p in [0, n),
  * written to host_out[4*p .. 4*p+3] (phase 1 = apply, 2 = refresh). */
 int lvbgs_debug_rng(lvbgs_ctx* ctx, uint32_t frame, uint32_t phase, uint32_t event, uint32_t n, uint32_t* host_out);

@@ -122,6 +122,7 @@ def lib():
         L.lvbgs_last_error.argtypes = [vp]
         L.lvbgs_last_error.restype = C.c_char_p
         L.lvbgs_lbsp_compute_abs.argtypes = [vp, i, i, i, i, vp, vp]
+        L.lvbgs_debug_lbsp_packed.argtypes = [vp, i, i, i, i, vp, vp]
         L.lvbgs_get_stats.argtypes = [vp, C.POINTER(Stats)]
         L.lvbgs_get_state.argtypes = [vp, C.c_char_p, vp, sz]
         L.lvbgs_get_state.restype = C.c_longlong
@@ -203,10 +204,11 @@ def _u8(img):
     return a
 
 
-def lbsp_compute_abs(d_img, width, height, channels, threshold, d_out, stream=None):
-    """LBSP::compute2 with an absolute threshold on device buffers (LBSP.cpp:102-123)."""
-    rc = lib().lvbgs_lbsp_compute_abs(C.c_void_p(d_img), width, height, channels, int(threshold), C.c_void_p(d_out),
-                                      C.c_void_p(stream or 0))
+def lbsp_compute_abs(d_img, width, height, channels, threshold, d_out, stream=None, packed=False):
+    """LBSP::compute2 with an absolute threshold on device buffers (LBSP.cpp:102-123). packed=True
+    runs it through the sample match's form of the threshold test (lvbgs_debug_lbsp_packed)."""
+    fn = lib().lvbgs_debug_lbsp_packed if packed else lib().lvbgs_lbsp_compute_abs
+    rc = fn(C.c_void_p(d_img), width, height, channels, int(threshold), C.c_void_p(d_out), C.c_void_p(stream or 0))
     if rc != OK:
         raise LvbgsError(rc, "lvbgs_lbsp_compute_abs failed")
 

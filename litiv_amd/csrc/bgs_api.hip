@@ -605,7 +605,15 @@ extern "C" int lvbgs_set_roi(lvbgs_ctx* c, uint8_t* roi, size_t stride) {
 extern "C" int lvbgs_lbsp_compute_abs(const void* d_img, int w, int h, int ch, int threshold, void* d_out, void* stream) {
     if (!d_img || !d_out || w <= 0 || h <= 0 || (ch != 1 && ch != 3)) return LVBGS_ERR_ARG;
     const uint32_t t = (uint32_t)(threshold < 0 ? 0 : threshold > 255 ? 255 : threshold);  // saturate_cast<uchar>
-    launch_lbsp_abs((const uint8_t*)d_img, w, h, ch, t, (uint16_t*)d_out, (hipStream_t)stream);
+    launch_lbsp_abs((const uint8_t*)d_img, w, h, ch, t, (uint16_t*)d_out, false, (hipStream_t)stream);
+    return hipGetLastError() == hipSuccess ? LVBGS_OK : LVBGS_ERR_HIP;
+}
+
+extern "C" int lvbgs_debug_lbsp_packed(const void* d_img, int w, int h, int ch, int threshold, void* d_out,
+                                       void* stream) {
+    if (!d_img || !d_out || w <= 0 || h <= 0 || (ch != 1 && ch != 3)) return LVBGS_ERR_ARG;
+    const uint32_t t = (uint32_t)(threshold < 0 ? 0 : threshold > 255 ? 255 : threshold);
+    launch_lbsp_abs((const uint8_t*)d_img, w, h, ch, t, (uint16_t*)d_out, true, (hipStream_t)stream);
     return hipGetLastError() == hipSuccess ? LVBGS_OK : LVBGS_ERR_HIP;
 }
 

@@ -60,8 +60,28 @@ LV_HD uint32_t mod64(uint32_t d, uint64_t m) {
 constexpr int kLbspDX[16] = {-2, 2, 0, 0, -2, 2, 2, -2, 0, -1, 0, 1, -1, 1, 1, -1};
 constexpr int kLbspDY[16] = {0, 0, -2, 2, 2, -2, 2, -2, 1, 0, -1, 0, -1, 1, -1, 1};
 
-// bit n = (|v[n]-ref| > t), LBSP.hpp:193-224. Packed taps: word w holds taps 4w..4w+3 (byte k = tap
-// 4w+k), four per VGPR. One v_mqsad_u32_u8 per word: window k of the quad SAD starts at byte k of
+// bit n = (|v[n]-ref| > t), LBSP.hpp:193-224. Two forms with the same bits; taps are passed packed
+// (word w holds taps 4w..4w+3, byte k = tap 4w+k) to both.
+//
+// lbsp_threshold_sad, for callers whose taps come byte by byte (the frame tile in LDS): one v_sad_u8
+// per tap. v_sad_u8(v, ref, ~t) = |v-ref| + 2^32-1-t wraps past 2^32 (bit 31 clears) exactly when
+// |v-ref| > t (both at most 255); v_alignbit shifts that bit 31 into the result, tap 15 first, so
+// bit n ends up holding tap n's "|v-ref| <= t"; the result is its complement. Two VALU per tap, one
+// VGPR per tap once the compiler keeps the bytes apart.
+LV_D uint32_t lbsp_threshold_sad(const uint32_t taps[4], uint32_t ref, uint32_t t) {
+    const uint32_t acc = ~t;
+    uint32_t d = 0;
+#pragma unroll
+    for (int n = 15; n >= 0; --n) {
+        const uint32_t v = (taps[n >> 2] >> (8 * (n & 3))) & 0xFFu;
+        const uint32_t x = __builtin_amdgcn_sad_u8(v, ref, acc);
+        d = __builtin_amdgcn_alignbit(d, x, 31u);  // (d << 1) | (x >> 31)
+    }
+    return ~d & 0xFFFFu;
+}
+
+// lbsp_threshold_packed, for the sample match, whose taps stay in registers across the whole scan:
+// the taps stay packed four per VGPR (a quarter of the registers). One v_mqsad_u32_u8 per word: window k of the quad SAD starts at byte k of
 // S0, so it sees tap 4w+k in its byte 0; S1 holds the reference in byte 0 and zeros elsewhere, and
 // the masked SAD skips the bytes where S1 is zero, so result k = |tap - ref| + ~t, which wraps past
 // 2^32 (bit 31 clears) exactly when |tap - ref| > t (both at most 255). v_alignbit shifts those bits

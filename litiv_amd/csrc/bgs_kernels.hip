@@ -696,7 +696,7 @@ LV_D void describe_body(const View& v, uint32_t bid, uint32_t nblk) {
                 uint32_t taps[4];
                 taps_from_tile<C>(tile, lx, ly, c, taps);
                 const uint32_t ref = tile.px[((ly + 2) * FrameTile<C>::LW + (lx + 2)) * C + c];
-                intra[c] = lbsp_threshold_packed(taps, ref, lut[ref]);
+                intra[c] = lbsp_threshold_sad(taps, ref, lut[ref]);
                 v.desc_cur[(size_t)p * C + c] = (uint16_t)intra[c];
                 pc += popc(intra[c]);
             }
@@ -1068,17 +1068,18 @@ template <int C, int ALGO, int KF>
 LV_D bool match_pixel(const View& v, const uint8_t* frame, const uint8_t* lut, const FrameTile<C>& tile,
                       const PredTile& pred, const PxIn<C>& in, int lx, int ly, int x, int y, uint32_t p,
                       uint32_t seq_limit, uint32_t& examined, PxCtx<C>& px) {
-    build_ctx_staged<C, ALGO, FrameTile<C>::LW * C>(v, &tile.px[(ly * FrameTile<C>::LW + lx) * C], in, px);
     // guess: raster-earlier writers decide as in the previous frame
     const uint32_t applied = applied_writers_tile(pred, lx, ly);
     uint64_t sbm;
     int nrep;
     ScanState st = scan_init<C>();
     const uint32_t N = (uint32_t)v.N;
-    // more than kRepl overwritten slots (very rare): the whole scan goes to the cooperative phase
+    // more than kRepl overwritten slots (very rare): the whole scan goes to the cooperative phase.
+    // The replacements are gathered before the context is built, so the two are not live together.
     auto rec_of = [&](int i) { return pred.rec[ly + 2 + kPredDY[i]][lx + 2 + kPredDX[i]]; };
     const uint32_t limit =
         build_repl<C>(v, frame, x, y, applied, sbm, nrep, rec_of) ? (seq_limit < N ? seq_limit : N) : 0u;
+    build_ctx_staged<C, ALGO, FrameTile<C>::LW * C>(v, &tile.px[(ly * FrameTile<C>::LW + lx) * C], in, px);
     scan_seq<C, ALGO, FrameTile<C>::LW * C, KF>(v, lut, px, x, y, sbm, nrep, st, limit);
     examined = st.s;
     v.mres[p] = pack_state(st);
@@ -2287,7 +2288,7 @@ LV_D void refresh_pixel(const View& v, uint32_t p, int use_pending, int force_fg
             if constexpr (ALGO == kLOBSTER) {
                 uint32_t taps[4];
                 taps_from_global<C>(v.last_color, v.W, sx, sy, c, taps);
-                desc[c] = lbsp_threshold_packed(taps, col[c], v.sc->lut[col[c]]);
+                desc[c] = lbsp_threshold_sad(taps, col[c], v.sc->lut[col[c]]);
             } else {
                 desc[c] = v.last_desc[(size_t)sp * C + c];
             }
@@ -2310,7 +2311,7 @@ __global__ void __launch_bounds__(TPB) k_init_desc(View v, const uint8_t* __rest
             uint32_t taps[4];
             taps_from_global<C>(img, v.W, x, y, c, taps);
             const uint32_t ref = img[(size_t)p * C + c];
-            d = (uint16_t)lbsp_threshold_packed(taps, ref, v.sc->lut[ref]);
+            d = (uint16_t)lbsp_threshold_sad(taps, ref, v.sc->lut[ref]);
         }
         v.last_desc[(size_t)p * C + c] = d;
         v.last_color[(size_t)p * C + c] = v.roi[p] ? img[(size_t)p * C + c] : (uint8_t)0;
@@ -2347,7 +2348,8 @@ __global__ void __launch_bounds__(TPB) k_background(View v, uint8_t* __restrict_
 }
 
 // stand-alone LBSP map (LBSP::compute2, LBSP.cpp:102-123): absolute threshold t
-template <int C>
+// LBSP with an absolute threshold over a whole image, through either form of the threshold test
+template <int C, bool PACKED>
 __global__ void __launch_bounds__(TPB) k_lbsp_abs(const uint8_t* __restrict__ img, int W, int H, uint32_t t,
                                                 uint16_t* __restrict__ out) {
     __shared__ FrameTile<C> tile;
@@ -2365,7 +2367,8 @@ __global__ void __launch_bounds__(TPB) k_lbsp_abs(const uint8_t* __restrict__ im
         if (in) {
             uint32_t taps[4];
             taps_from_tile<C>(tile, lx, ly, c, taps);
-            d = (uint16_t)lbsp_threshold_packed(taps, tile.px[((ly + 2) * FrameTile<C>::LW + (lx + 2)) * C + c], t);
+            const uint32_t ref = tile.px[((ly + 2) * FrameTile<C>::LW + (lx + 2)) * C + c];
+            d = (uint16_t)(PACKED ? lbsp_threshold_packed(taps, ref, t) : lbsp_threshold_sad(taps, ref, t));
         }
         out[((size_t)y * W + x) * C + c] = d;
     }
@@ -2630,10 +2633,15 @@ void launch_background(const View& v, uint8_t* out_img, uint16_t* out_desc, hipS
     else hipLaunchKernelGGL(k_background<1>, dim3(b), dim3(TPB), 0, st, v, out_img, out_desc);
 }
 
-void launch_lbsp_abs(const uint8_t* img, int W, int H, int C, uint32_t t, uint16_t* out, hipStream_t st) {
+void launch_lbsp_abs(const uint8_t* img, int W, int H, int C, uint32_t t, uint16_t* out, bool packed, hipStream_t st) {
     const unsigned tiles = cdiv(W, TW) * cdiv(H, TH);
-    if (C == 3) hipLaunchKernelGGL(k_lbsp_abs<3>, dim3(tiles), dim3(TPB), 0, st, img, W, H, t, out);
-    else hipLaunchKernelGGL(k_lbsp_abs<1>, dim3(tiles), dim3(TPB), 0, st, img, W, H, t, out);
+    if (packed) {
+        if (C == 3) hipLaunchKernelGGL((k_lbsp_abs<3, true>), dim3(tiles), dim3(TPB), 0, st, img, W, H, t, out);
+        else hipLaunchKernelGGL((k_lbsp_abs<1, true>), dim3(tiles), dim3(TPB), 0, st, img, W, H, t, out);
+    } else {
+        if (C == 3) hipLaunchKernelGGL((k_lbsp_abs<3, false>), dim3(tiles), dim3(TPB), 0, st, img, W, H, t, out);
+        else hipLaunchKernelGGL((k_lbsp_abs<1, false>), dim3(tiles), dim3(TPB), 0, st, img, W, H, t, out);
+    }
 }
 
 // the SuBSENSE hole-filling and median kernels alone, on whatever v.raw holds

@@ -52,7 +52,7 @@ void batch_step(const View* d_views, int ns, const uint32_t* const prefix[kNumGr
 void launch_refresh(const View& v, float frac, int force_fg, uint32_t frame, uint32_t ev, hipStream_t st);
 void launch_init_desc(const View& v, const uint8_t* img, hipStream_t st);
 void launch_background(const View& v, uint8_t* out_img, uint16_t* out_desc, hipStream_t st);
-void launch_lbsp_abs(const uint8_t* img, int W, int H, int C, uint32_t t, uint16_t* out, hipStream_t st);
+void launch_lbsp_abs(const uint8_t* img, int W, int H, int C, uint32_t t, uint16_t* out, bool packed, hipStream_t st);
 void launch_postprocess_probe(const View& v, hipStream_t st);
 // out[i] = the sticky error bits (Scalars::err_flags) of d_views[i], i < n
 void launch_gather_errors(const View* d_views, int n, uint32_t* d_out, hipStream_t st);

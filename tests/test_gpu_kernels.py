@@ -18,14 +18,14 @@ def _torch():
     return torch
 
 
-def _lbsp_gpu(img, t):
+def _lbsp_gpu(img, t, packed=False):
     torch = _torch()
     h, w = img.shape[:2]
     c = 1 if img.ndim == 2 else img.shape[2]
     d_img = torch.from_numpy(np.ascontiguousarray(img)).cuda()
     d_out = torch.empty((h, w, c), dtype=torch.int16, device="cuda")
     torch.cuda.synchronize()
-    lbsp_compute_abs(d_img.data_ptr(), w, h, c, t, d_out.data_ptr())
+    lbsp_compute_abs(d_img.data_ptr(), w, h, c, t, d_out.data_ptr(), packed=packed)
     torch.cuda.synchronize()
     return d_out.cpu().numpy().view(np.uint16).reshape(img.shape)
 
@@ -34,32 +34,34 @@ def test_lbsp_known_answer():
     # modules/features2d/test/lbsp.cpp:22-46: LBSP(20) on the img2.png crop vs test_lbsp.bin, valid zone
     inp = np.load(os.path.join(GOLDEN, "lbsp_kat_input.npy"))
     exp = np.load(os.path.join(GOLDEN, "lbsp_kat_expected.npy"))
-    out = _lbsp_gpu(inp, 20)
-    assert np.array_equal(out[2:63, 2:63], exp[2:63, 2:63])
+    for packed in (False, True):
+        out = _lbsp_gpu(inp, 20, packed)
+        assert np.array_equal(out[2:63, 2:63], exp[2:63, 2:63]), packed
 
 
 @pytest.mark.parametrize("shape,t", [((37, 53, 3), 0), ((64, 65, 1), 7), ((130, 200, 3), 35), ((5, 5, 1), 255)])
 def test_lbsp_random_vs_oracle(shape, t):
     rng = np.random.default_rng(sum(shape) + t)
     img = rng.integers(0, 256, size=shape, dtype=np.uint8)
-    out = _lbsp_gpu(img, t)
     ref = np.zeros(shape, np.uint16)
     orc().orc_lbsp_compute_abs(img, shape[1], shape[0], shape[2], t, ref)
-    assert np.array_equal(out, ref)
+    for packed in (False, True):
+        assert np.array_equal(_lbsp_gpu(img, t, packed), ref), packed
 
 
 @pytest.mark.parametrize("t", [0, 1, 2, 127, 253, 254, 255, 256])
 def test_lbsp_extreme_values_vs_oracle(t):
-    # centres and taps at the ends of the byte range: a zero centre (which the masked quad SAD
-    # treats through the complemented taps), differences of exactly t and t + 1, and 255 - 0
+    # centres and taps at the ends of the byte range: a zero centre (which the masked quad SAD of
+    # the packed form treats through the complemented taps), differences of exactly t and t + 1, and
+    # 255 - 0, through both forms of the threshold test
     rng = np.random.default_rng(1000 + t)
     vals = np.array([0, 1, 2, 126, 127, 128, 253, 254, 255], np.uint8)
     for shape in ((48, 70, 3), (33, 29, 1)):
         img = vals[rng.integers(0, len(vals), size=shape)]
-        out = _lbsp_gpu(img, t)
         ref = np.zeros(shape, np.uint16)
         orc().orc_lbsp_compute_abs(img, shape[1], shape[0], shape[2], t, ref)
-        assert np.array_equal(out, ref), shape
+        for packed in (False, True):
+            assert np.array_equal(_lbsp_gpu(img, t, packed), ref), (shape, packed)
 
 
 @pytest.mark.parametrize("kind,over", [(synth.QVGA, {}), (synth.THERMAL, {}), (synth.BATCH64, {"stream": 50})])
