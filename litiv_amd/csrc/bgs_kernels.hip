@@ -189,8 +189,8 @@ template <int C>
 LV_D void current_value(const View& v, const uint8_t* frame, uint32_t q, uint32_t col[C], uint32_t desc[C]) {
 #pragma unroll
     for (int c = 0; c < C; ++c) {
-        col[c] = frame[(size_t)q * C + c];
-        desc[c] = v.desc_cur[(size_t)q * C + c];
+        col[c] = G(frame)[(size_t)q * C + c];
+        desc[c] = G(v.desc_cur)[(size_t)q * C + c];
     }
 }
 
@@ -210,7 +210,7 @@ LV_D void load_frame_tile(FrameTile<C>& t, const uint8_t* frame, int W, int H, i
         const int ly = i / LW, lx = i % LW, gx = x0 + lx - 2, gy = y0 + ly - 2;
         const bool in = gx >= 0 && gx < W && gy >= 0 && gy < H;
 #pragma unroll
-        for (int c = 0; c < C; ++c) t.px[i * C + c] = in ? frame[((size_t)gy * W + gx) * C + c] : (uint8_t)0;
+        for (int c = 0; c < C; ++c) t.px[i * C + c] = in ? G(frame)[((size_t)gy * W + gx) * C + c] : (uint8_t)0;
     }
 }
 
@@ -238,7 +238,7 @@ LV_D void taps_from_global(const uint8_t* frame, int W, int x, int y, int c, uin
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             const int n = 4 * w + k;
-            acc |= (uint32_t)frame[((size_t)(y + kLbspDY[n]) * W + (x + kLbspDX[n])) * C + c] << (8 * k);
+            acc |= (uint32_t)G(frame)[((size_t)(y + kLbspDY[n]) * W + (x + kLbspDX[n])) * C + c] << (8 * k);
         }
         taps[w] = acc;
     }
@@ -380,7 +380,7 @@ LV_D uint32_t candidate_writers(const View& v, int x, int y) {
 #pragma unroll
     for (int i = 0; i < 12; ++i) {
         const uint32_t q = (uint32_t)((y + kPredDY[i]) * v.W + (x + kPredDX[i]));
-        const uint32_t r = v.rec[q];
+        const uint32_t r = G(v.rec)[q];
         const uint32_t code = (uint32_t)((2 - kPredDY[i]) * 5 + (2 - kPredDX[i]));
         if (rec_tgt(r) == code && fwd_gate(r)) m |= 1u << i;
     }
@@ -411,7 +411,7 @@ LV_D uint64_t applied_slots(const View& v, int x, int y, uint32_t applied) {
     for (uint32_t m = applied; m; m &= m - 1) {
         const int i = __builtin_ctz(m);
         const uint32_t q = (uint32_t)((y + kPredDY[i]) * v.W + (x + kPredDX[i]));
-        sbm |= 1ull << (rec_slot_diff(v.rec[q]) & 63u);
+        sbm |= 1ull << (rec_slot_diff(G(v.rec)[q]) & 63u);
     }
     return sbm;
 }
@@ -425,7 +425,7 @@ LV_D void sample_value(const View& v, const uint8_t* frame, int x, int y, uint32
         for (int i = 11; i >= 0; --i) {
             if (!((applied >> i) & 1u)) continue;
             const uint32_t q = (uint32_t)((y + kPredDY[i]) * v.W + (x + kPredDX[i]));
-            if (rec_slot_diff(v.rec[q]) == s) {
+            if (rec_slot_diff(G(v.rec)[q]) == s) {
                 current_value<C>(v, frame, q, col, desc);
                 break;
             }
@@ -638,7 +638,7 @@ LV_D bool forward_target(const View& v, uint32_t r, int x, int y, uint32_t& t) {
     const int dx = tgt_dx(tg), dy = tgt_dy(tg);
     if (!(dy > 0 || (dy == 0 && dx > 0))) return false;
     t = (uint32_t)((y + dy) * v.W + (x + dx));
-    return v.roi[t] != 0;
+    return G(v.roi)[t] != 0;
 }
 
 // ----------------------------------------------------------------------------------------------
@@ -672,13 +672,13 @@ LV_D void describe_body(const View& v, uint32_t bid, uint32_t nblk) {
     const bool inside = x < v.W && y < v.H;
     // the pixel's own state is loaded first, so these loads run alongside the tile's
     const uint32_t pp = inside ? (uint32_t)(y * v.W + x) : 0u;
-    const uint8_t roi_p = v.roi[pp];
+    const uint8_t roi_p = G(v.roi)[pp];
     float Rv = 0.f, Tv = 0.f, d_lt = 0.f, d_st = 0.f;
     if constexpr (ALGO == kSuBSENSE) {
-        Rv = v.R[pp];
-        Tv = v.T[pp];
-        d_lt = v.RawLT[pp] - v.FinLT[pp];
-        d_st = v.RawST[pp] - v.FinST[pp];
+        Rv = G(v.R)[pp];
+        Tv = G(v.T)[pp];
+        d_lt = G(v.RawLT)[pp] - G(v.FinLT)[pp];
+        d_st = G(v.RawST)[pp] - G(v.FinST)[pp];
     }
     load_frame_tile<C>(tile, frame, v.W, v.H, x0, y0);
     load_lut(lut, v.sc);
@@ -687,7 +687,7 @@ LV_D void describe_body(const View& v, uint32_t bid, uint32_t nblk) {
     if (inside) {
         const uint32_t p = pp;
         if (!roi_p) {
-            v.rec[p] = kRecNone;
+            G(v.rec)[p] = kRecNone;
         } else {
             uint32_t pc = 0;
             uint32_t intra[C];
@@ -697,7 +697,7 @@ LV_D void describe_body(const View& v, uint32_t bid, uint32_t nblk) {
                 taps_from_tile<C>(tile, lx, ly, c, taps);
                 const uint32_t ref = tile.px[((ly + 2) * FrameTile<C>::LW + (lx + 2)) * C + c];
                 intra[c] = lbsp_threshold_sad(taps, ref, lut[ref]);
-                v.desc_cur[(size_t)p * C + c] = (uint16_t)intra[c];
+                G(v.desc_cur)[(size_t)p * C + c] = (uint16_t)intra[c];
                 pc += popc(intra[c]);
             }
             nz = pc >= (C == 1 ? 2u : 4u);
@@ -727,7 +727,7 @@ LV_D void describe_body(const View& v, uint32_t bid, uint32_t nblk) {
                 bool ghost_fwd = false;
                 if (ty > y || (ty == y && tx > x)) {
                     const uint32_t t = (uint32_t)(ty * v.W + tx);
-                    ghost_fwd = v.RawST[t] > 0.995f && v.MLD[t] < 0.010f;
+                    ghost_fwd = G(v.RawST)[t] > 0.995f && G(v.MLD)[t] < 0.010f;
                 }
                 const uint32_t tg = (uint32_t)((ty - y + 2) * 5 + (tx - x + 2));
                 const uint32_t sd = (gate_a || gate_b) ? diff_slot() : 0u;
@@ -744,7 +744,7 @@ LV_D void describe_body(const View& v, uint32_t bid, uint32_t nblk) {
                 const uint32_t sd = gate_a ? diff_slot() : 0u;
                 r = tg | ((uint32_t)gate_a << 5) | ((uint32_t)self_bg << 8) | (mod64(d[1], N) << 16) | (sd << 24);
             }
-            v.rec[p] = r;
+            G(v.rec)[p] = r;
         }
     }
     if constexpr (ALGO == kSuBSENSE) block_add<TPB>(nz ? 1ull : 0ull, v.sc->nz_part);
@@ -756,8 +756,8 @@ LV_D void build_ctx(const View& v, const uint8_t* frame, const volatile uint8_t*
     const uint32_t p = (uint32_t)(y * v.W + x);
 #pragma unroll
     for (int c = 0; c < C; ++c) {
-        px.cur[c] = frame[(size_t)p * C + c];
-        px.intra[c] = v.desc_cur[(size_t)p * C + c];
+        px.cur[c] = G(frame)[(size_t)p * C + c];
+        px.intra[c] = G(v.desc_cur)[(size_t)p * C + c];
     }
 #pragma unroll
     for (int c = 0; c < C; ++c)
@@ -771,7 +771,7 @@ LV_D void build_ctx(const View& v, const uint8_t* frame, const volatile uint8_t*
             }
             px.taps[c][w] = acc;
         }
-    if constexpr (ALGO == kSuBSENSE) subsense_thresholds<C>(v, v.R[p], v.unstable[p], px);
+    if constexpr (ALGO == kSuBSENSE) subsense_thresholds<C>(v, G(v.R)[p], G(v.unstable)[p], px);
     else lobster_thresholds<C>(v, px);
 }
 
@@ -787,12 +787,12 @@ struct PxIn {
 template <int C, int ALGO>
 LV_D PxIn<C> load_px_in(const View& v, uint32_t q, bool in) {
     PxIn<C> r;
-    r.roi = in ? v.roi[q] : (uint8_t)0;
+    r.roi = in ? G(v.roi)[q] : (uint8_t)0;
 #pragma unroll
-    for (int c = 0; c < C; ++c) r.desc[c] = v.desc_cur[(size_t)q * C + c];
+    for (int c = 0; c < C; ++c) r.desc[c] = G(v.desc_cur)[(size_t)q * C + c];
     if constexpr (ALGO == kSuBSENSE) {
-        r.R = v.R[q];
-        r.unst = v.unstable[q];
+        r.R = G(v.R)[q];
+        r.unst = G(v.unstable)[q];
     } else {
         r.R = 0.f;
         r.unst = 0;
@@ -886,8 +886,8 @@ LV_D void build_ctx_global(const View& v, const uint8_t* frame, int x, int y, Px
     const uint32_t p = (uint32_t)(y * v.W + x);
 #pragma unroll
     for (int c = 0; c < C; ++c) {
-        px.cur[c] = frame[(size_t)p * C + c];
-        px.intra[c] = v.desc_cur[(size_t)p * C + c];
+        px.cur[c] = G(frame)[(size_t)p * C + c];
+        px.intra[c] = G(v.desc_cur)[(size_t)p * C + c];
     }
 #pragma unroll
     for (int c = 0; c < C; ++c)
@@ -897,11 +897,11 @@ LV_D void build_ctx_global(const View& v, const uint8_t* frame, int x, int y, Px
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
                 const int n = 4 * w + k;
-                acc |= (uint32_t)frame[((size_t)(y + kLbspDY[n]) * v.W + (x + kLbspDX[n])) * C + c] << (8 * k);
+                acc |= (uint32_t)G(frame)[((size_t)(y + kLbspDY[n]) * v.W + (x + kLbspDX[n])) * C + c] << (8 * k);
             }
             px.taps[c][w] = acc;
         }
-    if constexpr (ALGO == kSuBSENSE) subsense_thresholds<C>(v, v.R[p], v.unstable[p], px);
+    if constexpr (ALGO == kSuBSENSE) subsense_thresholds<C>(v, G(v.R)[p], G(v.unstable)[p], px);
     else lobster_thresholds<C>(v, px);
 }
 
@@ -910,7 +910,7 @@ template <int C>
 LV_D void stage_patch(const uint8_t* frame, int W, int x, int y, uint8_t* patch, int sub) {
     for (int j = sub; j < 25 * C; j += kGroup) {
         const int r = j / (5 * C), rem = j % (5 * C);
-        patch[j] = frame[((size_t)(y - 2 + r) * W + (x - 2)) * C + rem];
+        patch[j] = G(frame)[((size_t)(y - 2 + r) * W + (x - 2)) * C + rem];
     }
 }
 
@@ -936,8 +936,8 @@ LV_D void load_pred_tile(PredTile& t, const View& v, int x0, int y0) {
         const int ly = i / PredTile::LW, lx = i % PredTile::LW, gx = x0 + lx - 2, gy = y0 + ly - 2;
         const bool in = gx >= 0 && gx < v.W && gy >= 0 && gy < v.H;
         const size_t q = (size_t)gy * v.W + gx;
-        t.rec[ly][lx] = in ? v.rec[q] : kRecNone;
-        t.guess[ly][lx] = in ? (v.last_raw[q] ? 1 : 0) : 1;
+        t.rec[ly][lx] = in ? G(v.rec)[q] : kRecNone;
+        t.guess[ly][lx] = in ? (G(v.last_raw)[q] ? 1 : 0) : 1;
     }
 }
 
@@ -1054,7 +1054,7 @@ LV_D void match_body(const View& v, uint32_t bid, uint32_t nblk) {
         base = (uint32_t)__shfl((int)base, leader, 64);
         const uint32_t j = base + (uint32_t)__popcll(bal & ((1ull << lane) - 1ull));
         if (cont) {
-            v.cont[(size_t)k * v.cont_region + j] = p;
+            G(v.cont)[(size_t)k * v.cont_region + j] = p;
             if (j < v.cont_ctx_region)
                 store_ctx<C>(v.cont_ctx + ((size_t)k * v.cont_ctx_region + j) * kCtxWords, px);
         }
@@ -1082,13 +1082,13 @@ LV_D bool match_pixel(const View& v, const uint8_t* frame, const uint8_t* lut, c
     build_ctx_staged<C, ALGO, FrameTile<C>::LW * C>(v, &tile.px[(ly * FrameTile<C>::LW + lx) * C], in, px);
     scan_seq<C, ALGO, FrameTile<C>::LW * C, KF>(v, lut, px, x, y, sbm, nrep, st, limit);
     examined = st.s;
-    v.mres[p] = pack_state(st);
+    G(v.mres)[p] = pack_state(st);
     if (st.good < (uint32_t)v.REQ && st.s < N) return false;
     const uint8_t fg = st.good < (uint32_t)v.REQ ? 1 : 0;
-    v.dec[p] = fg;
-    const uint8_t guess = v.last_raw[p] ? 1 : 0;
+    G(v.dec)[p] = fg;
+    const uint8_t guess = G(v.last_raw)[p] ? 1 : 0;
     uint32_t t;
-    if (fg != guess && forward_target(v, v.rec[p], x, y, t)) wl_push(v, t, 1u, v.sc->epoch, v.wl1);
+    if (fg != guess && forward_target(v, G(v.rec)[p], x, y, t)) wl_push(v, t, 1u, v.sc->epoch, v.wl1);
     return true;
 }
 
@@ -1102,7 +1102,7 @@ LV_D uint32_t applied_writers_global(const View& v, int x, int y) {
         const int i = __builtin_ctz(m);
         const uint32_t q = (uint32_t)((y + kPredDY[i]) * v.W + (x + kPredDX[i]));
         const uint8_t d = CURRENT ? __hip_atomic_load(&v.dec[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
-                                  : (v.last_raw[q] ? 1 : 0);
+                                  : (G(v.last_raw)[q] ? 1 : 0);
         if (d == 0) applied |= 1u << i;
     }
     return applied;
@@ -1112,12 +1112,12 @@ LV_D uint32_t applied_writers_global(const View& v, int x, int y) {
 // decision differs from the speculative guess goes to the first fix-point worklist
 template <int C>
 LV_D void finish_pixel(const View& v, uint32_t p, int x, int y, const ScanState& st, uint64_t epoch) {
-    v.mres[p] = pack_state(st);
+    G(v.mres)[p] = pack_state(st);
     const uint8_t fg = st.good < (uint32_t)v.REQ ? 1 : 0;
-    v.dec[p] = fg;
-    const uint8_t guess = v.last_raw[p] ? 1 : 0;
+    G(v.dec)[p] = fg;
+    const uint8_t guess = G(v.last_raw)[p] ? 1 : 0;
     uint32_t t;
-    if (fg != guess && forward_target(v, v.rec[p], x, y, t)) wl_push(v, t, 1u, epoch, v.wl1);
+    if (fg != guess && forward_target(v, G(v.rec)[p], x, y, t)) wl_push(v, t, 1u, epoch, v.wl1);
 }
 
 // Phase B: the scans phase A cut at its sample limit, continued with one lane per pixel over the
@@ -1153,7 +1153,7 @@ LV_D void match_seq_body(const View& v, uint32_t bid, uint32_t nblk) {
         for (uint32_t half = kContStripes / 2; half > 0; half >>= 1)
             if (s_pre[k + half] <= i) k += half;
         const uint32_t j = i - s_pre[k];
-        const uint32_t p = v.cont[(size_t)k * v.cont_region + j];
+        const uint32_t p = G(v.cont)[(size_t)k * v.cont_region + j];
         const int x = (int)(p % (uint32_t)v.W), y = (int)(p / (uint32_t)v.W);
         PxCtx<C> px;
         if (j < v.cont_ctx_region) load_ctx<C>(v.cont_ctx + ((size_t)k * v.cont_ctx_region + j) * kCtxWords, px);
@@ -1161,12 +1161,12 @@ LV_D void match_seq_body(const View& v, uint32_t bid, uint32_t nblk) {
         const uint32_t applied = applied_writers_global<false>(v, x, y);
         uint64_t sbm;
         int nrep;
-        auto rec_of = [&](int i) { return v.rec[(size_t)(y + kPredDY[i]) * v.W + (x + kPredDX[i])]; };
+        auto rec_of = [&](int i) { return G(v.rec)[(size_t)(y + kPredDY[i]) * v.W + (x + kPredDX[i])]; };
         if (!build_repl<C>(v, frame, x, y, applied, sbm, nrep, rec_of)) {
             v.cont2[atomicAdd(&v.sc->cont2_count, 1u)] = p;
             continue;
         }
-        ScanState st = unpack_state(v.mres[p]);
+        ScanState st = unpack_state(G(v.mres)[p]);
         const uint32_t s0 = st.s;
         scan_seq<C, ALGO, 0, KF>(v, lut, px, x, y, sbm, nrep, st, (uint32_t)v.N);
         examined += st.s - s0;
@@ -1195,7 +1195,7 @@ LV_D void match_cont_body(const View& v, uint32_t bid, uint32_t nblk) {
     for (uint32_t r = 0; r < rounds; ++r) {
         const uint32_t i = g0 + r * ngroups;
         const bool active = i < n;
-        const uint32_t p = active ? v.cont2[i] : 0u;
+        const uint32_t p = active ? G(v.cont2)[i] : 0u;
         const int x = (int)(p % (uint32_t)v.W), y = (int)(p / (uint32_t)v.W);
         PxCtx<C> px;
         uint32_t applied = 0;
@@ -1207,7 +1207,7 @@ LV_D void match_cont_body(const View& v, uint32_t bid, uint32_t nblk) {
             build_ctx<C, ALGO, 5 * C>(v, frame, patch, x, y, px);
             applied = applied_writers_global<false>(v, x, y);
             sbm = applied_slots(v, x, y, applied);
-            st = unpack_state(v.mres[p]);
+            st = unpack_state(G(v.mres)[p]);
         }
         __syncthreads();
         const uint32_t s_before = st.s;
@@ -1241,13 +1241,13 @@ LV_D void rematch_round(const View& v, const uint8_t* frame, const uint8_t* lut,
     ScanState st = scan_init<C>();
     scan_coop<C, ALGO, 5 * C>(v, frame, lut, px, x, y, applied, sbm, st, active);
     if (active && (threadIdx.x & (kGroup - 1)) == 0) {
-        v.mres[p] = pack_state(st);
+        G(v.mres)[p] = pack_state(st);
         const uint8_t fg = st.good < (uint32_t)v.REQ ? 1 : 0;
-        const uint8_t old = v.dec[p];
+        const uint8_t old = G(v.dec)[p];
         if (fg != old) {
             __hip_atomic_store(&v.dec[p], fg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             uint32_t t;
-            if (forward_target(v, v.rec[p], x, y, t)) wl_push(v, t, k + 1u, epoch, wl_next);
+            if (forward_target(v, G(v.rec)[p], x, y, t)) wl_push(v, t, k + 1u, epoch, wl_next);
         }
     }
 }
@@ -1332,19 +1332,19 @@ LV_D uint32_t feedback_pixel(const View& v, uint32_t p, float& rst_out, float& m
     const uint8_t* frame = v.frame;
     uint32_t examined = 0;
     uint8_t raw = 0;
-    if (!v.roi[p]) {
-        rst_out = v.RawST[p];
-        mld_out = v.MLD[p];
+    if (!G(v.roi)[p]) {
+        rst_out = G(v.RawST)[p];
+        mld_out = G(v.MLD)[p];
     } else {
         const uint32_t frame_no = cur_frame(v);
         const uint32_t mvn = v.mv_avgs;
         const float aLT = 1.0f / (float)(frame_no < mvn ? frame_no : mvn);
         const float aST = 1.0f / (float)(frame_no < mvn / 4u ? frame_no : mvn / 4u);
-        const uint32_t mr = v.mres[p];
+        const uint32_t mr = G(v.mres)[p];
         const uint32_t good = mr & 255u, md = (mr >> 8) & 63u, ms = (mr >> 14) & 1023u;
         examined = mr >> 24;
-        const bool fg = v.dec[p] != 0;
-        const uint32_t r = v.rec[p];
+        const bool fg = G(v.dec)[p] != 0;
+        const uint32_t r = G(v.rec)[p];
         const uint32_t unst_new = rec_unstable(r) ? 1u : 0u;
         const float col_max = (float)(C * 255), desc_max = (float)(C * 16);
         // D_last (SUB:396-397): L1dist<3> accumulates in uchar (wraps mod 256)
@@ -1352,16 +1352,16 @@ LV_D uint32_t feedback_pixel(const View& v, uint32_t p, float& rst_out, float& m
         uint32_t cur[C], intra[C];
 #pragma unroll
         for (int c = 0; c < C; ++c) {
-            cur[c] = frame[(size_t)p * C + c];
-            intra[c] = v.desc_cur[(size_t)p * C + c];
-            l1 += absdiff(v.last_color[(size_t)p * C + c], cur[c]);
-            hd += popc((uint32_t)v.last_desc[(size_t)p * C + c] ^ intra[c]);
+            cur[c] = G(frame)[(size_t)p * C + c];
+            intra[c] = G(v.desc_cur)[(size_t)p * C + c];
+            l1 += absdiff(G(v.last_color)[(size_t)p * C + c], cur[c]);
+            hd += popc((uint32_t)G(v.last_desc)[(size_t)p * C + c] ^ intra[c]);
         }
         l1 &= 255u;
         const float nld = ((float)l1 / col_max + (float)hd / desc_max) / 2.0f;
-        const float mld = v.MLD[p] * (1.0f - aST) + nld * aST;
-        v.MLD[p] = mld;
-        float dlt = v.DminLT[p], dst = v.DminST[p], rlt = v.RawLT[p], rst = v.RawST[p];
+        const float mld = G(v.MLD)[p] * (1.0f - aST) + nld * aST;
+        G(v.MLD)[p] = mld;
+        float dlt = G(v.DminLT)[p], dst = G(v.DminST)[p], rlt = G(v.RawLT)[p], rst = G(v.RawST)[p];
         const uint32_t REQ = (uint32_t)v.REQ;
         if (fg) {
             const float nmd = fmin_ref(1.0f, ((float)ms / col_max + (float)md / desc_max) / 2.0f +
@@ -1378,16 +1378,16 @@ LV_D uint32_t feedback_pixel(const View& v, uint32_t p, float& rst_out, float& m
             rlt = rlt * (1.0f - aLT);
             rst = rst * (1.0f - aST);
         }
-        v.DminLT[p] = dlt;
-        v.DminST[p] = dst;
-        v.RawLT[p] = rlt;
-        v.RawST[p] = rst;
+        G(v.DminLT)[p] = dlt;
+        G(v.DminST)[p] = dst;
+        G(v.RawLT)[p] = rlt;
+        G(v.RawST)[p] = rst;
         rst_out = rst;
         mld_out = mld;
         const float lcap = v.sc->lower_cap, ucap = v.sc->upper_cap;
         const float mn = fmin_ref(dlt, dst), mx = fmax_ref(dlt, dst);
-        const bool last_fg = v.last_fg[p] != 0;
-        float T = v.T[p], V = v.V[p], R = v.R[p];
+        const bool last_fg = G(v.last_fg)[p] != 0;
+        float T = G(v.T)[p], V = G(v.V)[p], R = G(v.R)[p];
         if (last_fg || (mn < 0.1f && raw)) {
             if (T < ucap) T += 0.5f / (mx * V);
         } else if (T > lcap) {
@@ -1395,7 +1395,7 @@ LV_D uint32_t feedback_pixel(const View& v, uint32_t p, float& rst_out, float& m
         }
         if (T < lcap) T = lcap;
         else if (T > ucap) T = ucap;
-        if (mx > 0.1f && v.blinks[p]) {
+        if (mx > 0.1f && G(v.blinks)[p]) {
             V += 1.0f;
         } else if (V > 0.1f) {
             V -= last_fg ? 0.1f / 4 : unst_new ? 0.1f / 2 : 0.1f;
@@ -1408,21 +1408,21 @@ LV_D uint32_t feedback_pixel(const View& v, uint32_t p, float& rst_out, float& m
             R -= 0.01f / V;
             if (R < 1.0f) R = 1.0f;
         }
-        v.T[p] = T;
-        v.V[p] = V;
-        v.R[p] = R;
-        v.unstable[p] = (uint8_t)unst_new;
+        G(v.T)[p] = T;
+        G(v.V)[p] = V;
+        G(v.R)[p] = R;
+        G(v.unstable)[p] = (uint8_t)unst_new;
 #pragma unroll
         for (int c = 0; c < C; ++c) {
-            v.last_color[(size_t)p * C + c] = (uint8_t)cur[c];
-            v.last_desc[(size_t)p * C + c] = (uint16_t)intra[c];
+            G(v.last_color)[(size_t)p * C + c] = (uint8_t)cur[c];
+            G(v.last_desc)[(size_t)p * C + c] = (uint16_t)intra[c];
         }
     }
-    const uint8_t cb = raw ^ v.last_raw[p];
-    v.blinks[p] = cb | v.last_raw_blink[p];
-    v.last_raw_blink[p] = cb;
-    v.last_raw[p] = raw;
-    v.raw[p] = raw;
+    const uint8_t cb = raw ^ G(v.last_raw)[p];
+    G(v.blinks)[p] = cb | G(v.last_raw_blink)[p];
+    G(v.last_raw_blink)[p] = cb;
+    G(v.last_raw)[p] = raw;
+    G(v.raw)[p] = raw;
     return examined;
 }
 
@@ -1432,12 +1432,12 @@ LV_D void lobster_raw_body(const View& v, uint32_t bid, uint32_t nblk) {
     uint32_t examined = 0;
     if (p < v.npx) {
         uint8_t raw = 0;
-        if (v.roi[p]) {
-            raw = v.dec[p] ? 255 : 0;
-            examined = v.mres[p] >> 24;
+        if (G(v.roi)[p]) {
+            raw = G(v.dec)[p] ? 255 : 0;
+            examined = G(v.mres)[p] >> 24;
         }
-        v.raw[p] = raw;
-        v.last_raw[p] = raw;
+        G(v.raw)[p] = raw;
+        G(v.last_raw)[p] = raw;
     }
     block_add<TPB>(examined, v.sc->ex_part);
 }
@@ -1470,12 +1470,12 @@ LV_D void update_body(const View& v, uint32_t bid, uint32_t nblk) {
         const int ly = i / LW, lx = i % LW, gx = x0 + lx - 2, gy = y0 + ly - 2;
         const bool in = i < LW * LH && gx >= 0 && gx < v.W && gy >= 0 && gy < v.H;
         const size_t q = in ? (size_t)gy * v.W + gx : 0;
-        t_rec[k] = in ? v.rec[q] : kRecNone;
-        t_dec[k] = v.dec[q];
+        t_rec[k] = in ? G(v.rec)[q] : kRecNone;
+        t_dec[k] = G(v.dec)[q];
 #pragma unroll
         for (int c = 0; c < C; ++c) {
-            t_col[k][c] = frame[q * C + c];
-            t_desc[k][c] = v.desc_cur[q * C + c];
+            t_col[k][c] = G(frame)[q * C + c];
+            t_desc[k][c] = G(v.desc_cur)[q * C + c];
         }
     }
     const int lx = threadIdx.x % TW, ly = threadIdx.x / TW, x = x0 + lx, y = y0 + ly;
@@ -1502,7 +1502,7 @@ LV_D void update_body(const View& v, uint32_t bid, uint32_t nblk) {
         // ghost test of writers at or after p reads p's updated maps (SUB:438-441)
         bool ghost_new = false;
         if constexpr (FUSED) ghost_new = rst > 0.995f && mld < 0.010f;
-        else if constexpr (ALGO == kSuBSENSE) ghost_new = v.RawST[p] > 0.995f && v.MLD[p] < 0.010f;
+        else if constexpr (ALGO == kSuBSENSE) ghost_new = G(v.RawST)[p] > 0.995f && G(v.MLD)[p] < 0.010f;
         // Writes landing on p, as bits in sequential order: writer j of the 5x5 window (raster
         // order) owns bit 2j (its self update, j == 12 only) and bit 2j+1 (its diffusion into p).
         uint64_t wm = 0;
@@ -1553,7 +1553,7 @@ template <int C>
 LV_D void copy_last_color_body(const View& v, uint32_t bid, uint32_t nblk) {
     const uint8_t* frame = v.frame;
     const uint32_t i = bid * TPB + threadIdx.x;
-    if (i < v.npx * (uint32_t)C) v.last_color[i] = frame[i];
+    if (i < v.npx * (uint32_t)C) G(v.last_color)[i] = G(frame)[i];
 }
 
 // ----------------------------------------------------------------------------------------------
@@ -2046,11 +2046,11 @@ LV_D void dilate_final_body(const View& v, uint32_t bid, uint32_t nblk) {
             for (int u = 0; u < 4 && gx + u < W; ++u) {
                 const size_t pu = p + u;
                 const uint8_t iu = (uint8_t)(inv >> (8 * u)), fu = (uint8_t)(fg >> (8 * u));
-                v.blinks[pu] = (uint8_t)(v.blinks[pu] & v.last_dil_inv[pu] & iu);
-                v.last_dil_inv[pu] = iu;
+                G(v.blinks)[pu] = (uint8_t)(G(v.blinks)[pu] & G(v.last_dil_inv)[pu] & iu);
+                G(v.last_dil_inv)[pu] = iu;
                 const double f = (double)fu;
-                v.FinLT[pu] = (float)((double)v.FinLT[pu] * alt + f * blt + 0.0);
-                v.FinST[pu] = (float)((double)v.FinST[pu] * ast + f * bst + 0.0);
+                G(v.FinLT)[pu] = (float)((double)G(v.FinLT)[pu] * alt + f * blt + 0.0);
+                G(v.FinST)[pu] = (float)((double)G(v.FinST)[pu] * ast + f * bst + 0.0);
                 if (out_mask) out_mask[pu] = fu;
             }
         }
@@ -2061,7 +2061,7 @@ LV_D void copy_mask_body(const View& v, uint32_t bid, uint32_t nblk) {
     uint8_t* out_mask = v.out_mask;
     if (!out_mask) return;
     const uint32_t p = bid * TPB + threadIdx.x;
-    if (p < v.npx) out_mask[p] = v.last_fg[p];
+    if (p < v.npx) out_mask[p] = G(v.last_fg)[p];
 }
 
 // ----------------------------------------------------------------------------------------------
@@ -2143,7 +2143,7 @@ LV_D void downsample_body(const View& v, uint32_t bid, uint32_t nblk) {
                 const int sx = v.W / v.dsW, sy = v.H / v.dsH;
                 int sum = 0;
                 for (int yy = 0; yy < sy; ++yy)
-                    for (int xx = 0; xx < sx; ++xx) sum += frame[((size_t)(dy * sy + yy) * v.W + (dx * sx + xx)) * C + c];
+                    for (int xx = 0; xx < sx; ++xx) sum += G(frame)[((size_t)(dy * sy + yy) * v.W + (dx * sx + xx)) * C + c];
                 m = sat_u8((float)sum * (1.f / (float)(sx * sy)));
             } else {
                 const double sx = (double)v.W / v.dsW, sy = (double)v.H / v.dsH;
@@ -2153,7 +2153,7 @@ LV_D void downsample_body(const View& v, uint32_t bid, uint32_t nblk) {
                     const double wy = fmin((double)(yy + 1), ye) - fmax((double)yy, yb);
                     for (int xx = (int)xb; xx < v.W && (double)xx < xe; ++xx) {
                         const double wx = fmin((double)(xx + 1), xe) - fmax((double)xx, xb);
-                        acc += (double)frame[((size_t)yy * v.W + xx) * C + c] * (wy * wx);
+                        acc += (double)G(frame)[((size_t)yy * v.W + xx) * C + c] * (wy * wx);
                     }
                 }
                 m = sat_u8((float)(acc / (sx * sy)));
@@ -2269,9 +2269,9 @@ LV_D void refresh_body(const View& v, uint32_t bid, uint32_t nblk, int use_pendi
 template <int C, int ALGO>
 LV_D void refresh_pixel(const View& v, uint32_t p, int use_pending, int force_fg, uint32_t frame, uint32_t ev,
                         uint32_t nref, uint32_t start) {
-    if (use_pending && ALGO == kSuBSENSE) v.T[p] = 1.0f;
-    if (!v.roi[p]) return;
-    if (!(force_fg || !v.last_fg[p])) return;
+    if (use_pending && ALGO == kSuBSENSE) G(v.T)[p] = 1.0f;
+    if (!G(v.roi)[p]) return;
+    if (!(force_fg || !G(v.last_fg)[p])) return;
     const uint32_t N = (uint32_t)v.N;
     const int x = (int)(p % (uint32_t)v.W), y = (int)(p / (uint32_t)v.W);
     uint32_t d[4];
@@ -2280,17 +2280,17 @@ LV_D void refresh_pixel(const View& v, uint32_t p, int use_pending, int force_fg
         int sx, sy;
         sample_pos_7x7(d[j & 3u], x, y, v.W, v.H, sx, sy);
         const uint32_t sp = (uint32_t)(sy * v.W + sx);
-        if (!(force_fg || !v.last_fg[sp])) continue;
+        if (!(force_fg || !G(v.last_fg)[sp])) continue;
         uint32_t col[C], desc[C];
 #pragma unroll
         for (int c = 0; c < C; ++c) {
-            col[c] = v.last_color[(size_t)sp * C + c];
+            col[c] = G(v.last_color)[(size_t)sp * C + c];
             if constexpr (ALGO == kLOBSTER) {
                 uint32_t taps[4];
                 taps_from_global<C>(v.last_color, v.W, sx, sy, c, taps);
                 desc[c] = lbsp_threshold_sad(taps, col[c], v.sc->lut[col[c]]);
             } else {
-                desc[c] = v.last_desc[(size_t)sp * C + c];
+                desc[c] = G(v.last_desc)[(size_t)sp * C + c];
             }
         }
         store_sample<C>(v, (start + j) % N, p, col, desc);
@@ -2303,7 +2303,7 @@ __global__ void __launch_bounds__(TPB) k_init_desc(View v, const uint8_t* __rest
     const uint32_t p = blockIdx.x * TPB + threadIdx.x;
     if (p >= v.npx) return;
     const int x = (int)(p % (uint32_t)v.W), y = (int)(p / (uint32_t)v.W);
-    const bool in = v.roi[p] && x > 2 && y > 2 && x < v.W - 2 && y < v.H - 2;
+    const bool in = G(v.roi)[p] && x > 2 && y > 2 && x < v.W - 2 && y < v.H - 2;
 #pragma unroll
     for (int c = 0; c < C; ++c) {
         uint16_t d = 0;
@@ -2313,8 +2313,8 @@ __global__ void __launch_bounds__(TPB) k_init_desc(View v, const uint8_t* __rest
             const uint32_t ref = img[(size_t)p * C + c];
             d = (uint16_t)lbsp_threshold_sad(taps, ref, v.sc->lut[ref]);
         }
-        v.last_desc[(size_t)p * C + c] = d;
-        v.last_color[(size_t)p * C + c] = v.roi[p] ? img[(size_t)p * C + c] : (uint8_t)0;
+        G(v.last_desc)[(size_t)p * C + c] = d;
+        G(v.last_color)[(size_t)p * C + c] = G(v.roi)[p] ? img[(size_t)p * C + c] : (uint8_t)0;
     }
 }
 
