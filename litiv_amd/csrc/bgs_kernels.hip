@@ -178,9 +178,9 @@ LV_D void store_sample(const View& v, uint32_t s, uint32_t p, const uint32_t col
         w.x = (uint32_t)ra;
         w.y = (uint32_t)(ra >> 32);
         w.z = rb;
-        __builtin_nontemporal_store(w, (u32x3*)((Rec12*)v.sA + idx));
+        __builtin_nontemporal_store(w, (gptr<u32x3>)((Rec12*)v.sA + idx));
     } else {
-        __builtin_nontemporal_store((uint32_t)ra, (uint32_t*)v.sA + idx);
+        __builtin_nontemporal_store((uint32_t)ra, G((uint32_t*)v.sA + idx));
     }
 }
 
@@ -245,7 +245,8 @@ LV_D void taps_from_global(const uint8_t* frame, int W, int x, int y, int c, uin
 }
 
 LV_D void load_lut(uint8_t* lut_lds, const Scalars* sc) {
-    for (int i = threadIdx.x; i < 256; i += blockDim.x) lut_lds[i] = sc->lut[i];
+    const gptr<const uint8_t> src = G(sc->lut);
+    for (int i = threadIdx.x; i < 256; i += blockDim.x) lut_lds[i] = src[i];
 }
 
 // ----------------------------------------------------------------------------------------------
@@ -800,9 +801,10 @@ LV_D PxIn<C> load_px_in(const View& v, uint32_t q, bool in) {
     return r;
 }
 
-// the match context from the LDS frame tile (5x5 window at win) and the staged inputs
+// the match context from the LDS frame tile (5x5 window at win; the tile is complete behind a
+// workgroup barrier, so plain loads) and the staged inputs
 template <int C, int ALGO, int RS>
-LV_D void build_ctx_staged(const View& v, const volatile uint8_t* win, const PxIn<C>& in, PxCtx<C>& px) {
+LV_D void build_ctx_staged(const View& v, const uint8_t* win, const PxIn<C>& in, PxCtx<C>& px) {
 #pragma unroll
     for (int c = 0; c < C; ++c) {
         px.cur[c] = win[2 * RS + 2 * C + c];
@@ -845,18 +847,20 @@ LV_D void store_ctx(uint32_t* d, const PxCtx<C>& px) {
     w[17] = px.t_tot_desc;
     w[18] = px.t_col;
     w[19] = px.t_desc;
-    uint4* q = (uint4*)d;
+    typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+    const gptr<u32x4> q = (gptr<u32x4>)d;
 #pragma unroll
-    for (int k = 0; k < kCtxWords / 4; ++k) q[k] = make_uint4(w[4 * k], w[4 * k + 1], w[4 * k + 2], w[4 * k + 3]);
+    for (int k = 0; k < kCtxWords / 4; ++k) q[k] = u32x4{w[4 * k], w[4 * k + 1], w[4 * k + 2], w[4 * k + 3]};
 }
 
 template <int C>
 LV_D void load_ctx(const uint32_t* d, PxCtx<C>& px) {
     uint32_t w[kCtxWords];
-    const uint4* q = (const uint4*)d;
+    typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+    const gptr<const u32x4> q = (gptr<const u32x4>)d;
 #pragma unroll
     for (int k = 0; k < kCtxWords / 4; ++k) {
-        const uint4 u = q[k];
+        const u32x4 u = q[k];
         w[4 * k] = u.x;
         w[4 * k + 1] = u.y;
         w[4 * k + 2] = u.z;
