@@ -99,6 +99,13 @@ LV_D gptr<T> G(T* p) {
     return (gptr<T>)p;
 }
 
+// wavefront-local LDS exchange: the lanes' stores above are visible to the loads below
+LV_D void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // raw sample in registers: ra = record words 0-1 (RGB) or word 0 (grey), rb = word 2 (RGB)
 template <int C>
 LV_D void fetch_raw(const View& v, uint32_t s, uint32_t p, uint64_t& ra, uint32_t& rb) {
@@ -753,7 +760,7 @@ LV_D void describe_body(const View& v, uint32_t bid, uint32_t nblk) {
 
 // builds the match context of ROI pixel (x,y); taps from the LDS tile or from global memory
 template <int C, int ALGO, int RS>
-LV_D void build_ctx(const View& v, const uint8_t* frame, const volatile uint8_t* win, int x, int y, PxCtx<C>& px) {
+LV_D void build_ctx(const View& v, const uint8_t* frame, const uint8_t* win, int x, int y, PxCtx<C>& px) {
     const uint32_t p = (uint32_t)(y * v.W + x);
 #pragma unroll
     for (int c = 0; c < C; ++c) {
@@ -1208,6 +1215,7 @@ LV_D void match_cont_body(const View& v, uint32_t bid, uint32_t nblk) {
         __syncthreads();  // previous round's readers of the patch are done
         if (active) {
             stage_patch<C>(frame, v.W, x, y, patch, (int)sub);
+            wave_sync();  // the group's patch bytes, written by its eight lanes
             build_ctx<C, ALGO, 5 * C>(v, frame, patch, x, y, px);
             applied = applied_writers_global<false>(v, x, y);
             sbm = applied_slots(v, x, y, applied);
@@ -1237,6 +1245,7 @@ LV_D void rematch_round(const View& v, const uint8_t* frame, const uint8_t* lut,
     __syncthreads();  // previous round's readers of the patch are done
     if (active) {
         stage_patch<C>(frame, v.W, x, y, patch, (int)(threadIdx.x & (kGroup - 1)));
+        wave_sync();  // the group's patch bytes, written by its eight lanes
         build_ctx<C, ALGO, 5 * C>(v, frame, patch, x, y, px);
         applied = applied_writers_global<true>(v, x, y);
         sbm = applied_slots(v, x, y, applied);
@@ -1591,12 +1600,6 @@ LV_D uint64_t from_lane_above(uint64_t x, int d, int lane) {  // lane - d, 0 for
 LV_D uint64_t from_lane_below(uint64_t x, int d, int lane) {  // lane + d, 0 for the last d lanes
     const uint64_t r = __shfl_down(x, (unsigned)d, 64);
     return lane + d < 64 ? r : 0ull;
-}
-// wavefront-local LDS exchange: the lanes' stores above are visible to the loads below
-LV_D void wave_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 // the runs of z along the row that hold a bit of s (s within z). Adding s to z carries through
