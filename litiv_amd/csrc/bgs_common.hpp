@@ -81,13 +81,14 @@ LV_D uint32_t lbsp_threshold_sad(const uint32_t taps[4], uint32_t ref, uint32_t 
 }
 
 // lbsp_threshold_packed, for the sample match, whose taps stay in registers across the whole scan:
-// the taps stay packed four per VGPR (a quarter of the registers). One v_mqsad_u32_u8 per word: window k of the quad SAD starts at byte k of
-// S0, so it sees tap 4w+k in its byte 0; S1 holds the reference in byte 0 and zeros elsewhere, and
-// the masked SAD skips the bytes where S1 is zero, so result k = |tap - ref| + ~t, which wraps past
-// 2^32 (bit 31 clears) exactly when |tap - ref| > t (both at most 255). v_alignbit shifts those bits
-// in, tap 15 first, so bit n ends up holding tap n's "|v-ref| <= t"; the result is its complement.
-// The mask would also skip byte 0 when ref == 0: that case compares the complemented taps with 255
-// instead (255 - v against 255 - 0 has the same absolute difference).
+// they stay packed four per VGPR (a quarter of the registers). One v_mqsad_u32_u8 per word: window
+// k of the quad SAD starts at byte k of S0, so it sees tap 4w+k in its byte 0; S1 holds the
+// reference in byte 0 and zeros elsewhere, and the masked SAD skips the bytes where S1 is zero, so
+// result k = |tap - ref| + ~t, which wraps past 2^32 (bit 31 clears) exactly when |tap - ref| > t
+// (both at most 255). v_alignbit shifts those bits in, tap 15 first, so bit n ends up holding tap
+// n's "|v-ref| <= t"; the result is its complement. The mask would also skip byte 0 when ref == 0:
+// that case compares the complemented taps with 255 instead (255 - v against 255 - 0 has the same
+// absolute difference).
 LV_D uint32_t lbsp_threshold_packed(const uint32_t taps[4], uint32_t ref, uint32_t t) {
     typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
     const uint32_t acc = ~t;

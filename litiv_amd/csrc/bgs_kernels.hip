@@ -2150,7 +2150,8 @@ LV_D void downsample_body(const View& v, uint32_t bid, uint32_t nblk) {
                 const int sx = v.W / v.dsW, sy = v.H / v.dsH;
                 int sum = 0;
                 for (int yy = 0; yy < sy; ++yy)
-                    for (int xx = 0; xx < sx; ++xx) sum += G(frame)[((size_t)(dy * sy + yy) * v.W + (dx * sx + xx)) * C + c];
+                    for (int xx = 0; xx < sx; ++xx)
+                        sum += G(frame)[((size_t)(dy * sy + yy) * v.W + (dx * sx + xx)) * C + c];
                 m = sat_u8((float)sum * (1.f / (float)(sx * sy)));
             } else {
                 const double sx = (double)v.W / v.dsW, sy = (double)v.H / v.dsH;
@@ -2474,7 +2475,9 @@ __global__ void __launch_bounds__(TPB) k_refresh(View v, int use_pending, float 
     })
 __global__ void __launch_bounds__(TPB) kb_describe(BatchArgs b) { LV_BATCH_C(describe_body, kSuBSENSE); }
 template <int KF>
-__global__ void __launch_bounds__(TPB) LV_MATCH_WAVES(KF) kb_match(BatchArgs b) { LV_BATCH_C(match_body, kSuBSENSE, KF, 3); }
+__global__ void __launch_bounds__(TPB) LV_MATCH_WAVES(KF) kb_match(BatchArgs b) {
+    LV_BATCH_C(match_body, kSuBSENSE, KF, 3);
+}
 template <int KF>
 __global__ void __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(6))) kb_match_seq(BatchArgs b) {
     LV_BATCH_C(match_seq_body, kSuBSENSE, KF);
